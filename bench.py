@@ -1,0 +1,195 @@
+#!/usr/bin/env python3
+"""Throughput of the fused FP64 DG advection RK stage on one MI355X.
+
+One "step" is one SSP-RK3 time step (three fused stage launches, replayed
+from a hipGraph) of the periodic DG advection on the workload's mesh, with
+the state resident in HBM.  The metric is GDOF-updates per second per RK
+stage: Ne*Np*3*steps / seconds.
+
+Prints one JSON line.  Beside the GPU figure it reports the stage kernel's
+HBM roofline fraction (algorithmic 48 B per DOF-update: read q,u,v,w and q0
+or write q0, write q) and a CPU baseline: the reference's own Fortran
+modules (oracle/_ref, compiled from source) timed on the host cores over a
+bounded sample of the workload.
+
+    python bench.py [--steps K] [--warmup W] [--workload c3_64cube_p7]
+                    [--variant exact|fma] [--no-cpu-baseline]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "scale-dg-kernel-extraction_amd"))
+
+import dgk  # noqa: E402
+from dgk import ic  # noqa: E402
+
+METRIC = "GDOF-updates/s per RK stage at p=7, 64^3 elems; rel-L2 err vs FP64 oracle"
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_DOF = 48  # per DOF-update and stage, per-node FP64 velocity
+PROFILES = os.path.join(REPO, "profiles")
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def lgl_weights(x1D):
+    p = len(x1D) - 1
+    P = np.polynomial.legendre.Legendre.basis(p)(x1D)
+    return 2.0 / (p * (p + 1) * P * P)
+
+
+def mass(q, w3, J):
+    return float(J * np.dot(w3, q.sum(axis=1)))
+
+
+def cpu_baseline(threads, nstep):
+    """Times the reference's step loop (compiled Fortran, oracle/_ref) on a
+    32^3-element p=7 sample of the same operator and velocity; falls back to
+    the C restatement (oracle/liboracle.so) where the reference binary is
+    absent."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle as O
+    spec = ic.WORKLOADS["c3_64cube_p7"]
+    ne = (32, 32, 32)
+    p = spec["p"]
+    m = dgk.mesh_setup(*ne, p, tables=False)
+    q = ic.initial_field(spec["ic"], m["pos_en"], spec["seed"])
+    del m
+    vel = spec["vel"]
+    u, v, w = (np.full_like(q, c) for c in vel)
+    dt = ic.cfl_dt(p, vel, ne)
+    dof_updates = q.size * 3 * nstep
+    sample = f"32^3 elements, p=7, {nstep} steps ({dof_updates / 1e6:.1f} M DOF-updates)"
+    if O.ref_available():
+        _, _, sec, _ = O.ref_run(*ne, p, q, u, v, w, dt, nstep, "OPT1", threads=threads)
+        kind = "reference"
+    else:
+        os.environ["OMP_NUM_THREADS"] = str(threads)
+        om = O.Mesh(*ne, p)
+        F = []
+        for a in (q, u, v, w):
+            f = om.field()
+            f[:, :om.Ne] = a
+            om.update_halo(f)
+            F.append(f)
+        sec = om.run(*F, dt, nstep, O.OPT1)
+        kind = "port"
+    return {"value": dof_updates / sec / 1e9, "unit": "GDOF-updates/s", "cores": threads,
+            "kind": kind, "sample": sample, "seconds": sec}
+
+
+def traffic_from_profiles(workload, variant):
+    path = os.path.join(PROFILES, f"hbm_traffic_{workload}_{variant}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    return t.get("bytes_per_launch")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="c3_64cube_p7", choices=sorted(ic.WORKLOADS))
+    ap.add_argument("--variant", default="exact", choices=("exact", "fma"))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-steps", type=int, default=5)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
+              "sharding of the mesh is a design, not a measured path", file=sys.stderr)
+        sys.exit(2)
+
+    spec = ic.WORKLOADS[args.workload]
+    ne, p, vel = spec["ne"], spec["p"], spec["vel"]
+    log(f"workload {args.workload}: {ne} elements, p={p}, variant {args.variant}")
+    x1D, D, L = dgk.read_operator(p)
+    t0 = time.perf_counter()
+    m = dgk.mesh_setup(*ne, p, x1D=x1D, tables=False)
+    q = ic.initial_field(spec["ic"], m["pos_en"], spec["seed"])
+    del m
+    u, v, w = (np.full_like(q, c) for c in vel)
+    dt = ic.cfl_dt(p, vel, ne)
+    log(f"initial field generated in {time.perf_counter() - t0:.1f} s")
+    flags = dgk.FLAG_FMA if args.variant == "fma" else 0
+    s = dgk.Solver.structured(*ne, p, D, L, flags=flags)
+    s.set_fields(q, u, v, w)
+    w3 = np.einsum("i,j,k->ijk", *(lgl_weights(x1D),) * 3).ravel(order="F")
+    J = (1.0 / ne[0]) * (1.0 / ne[1]) * (1.0 / ne[2]) / 8.0
+    mass0 = mass(q, w3, J)
+    del q, u, v, w
+    Ne, Np = s.Ne, s.Np
+
+    for _ in range(args.warmup):
+        s.step(dt)
+    s.synchronize()
+    log(f"warmup done ({args.warmup} steps)")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s.step(dt)
+    s.synchronize()
+    el = time.perf_counter() - t0
+    dof = Ne * Np
+    value = dof * 3 * args.steps / el / 1e9
+    log(f"timed {args.steps} steps in {el:.4f} s -> {value:.2f} GDOF-updates/s")
+
+    # dominant kernel: the fused stage launch, timed with HIP events on the
+    # solver's own stream
+    nstage = max(30, 3 * args.steps)
+    ms_stage = s.time_stages(dt, nstage)
+    bytes_launch = BYTES_PER_DOF * dof
+    achieved = bytes_launch / (ms_stage * 1e-3) / 1e9
+    qf = s.get_field()
+    finite = bool(np.isfinite(qf).all())
+    drift = abs(mass(qf, w3, J) - mass0) / abs(mass0) if mass0 else None
+    s.close()
+    log(f"stage kernel {ms_stage * 1e3:.1f} us/launch -> {achieved:.0f} GB/s algorithmic")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        log(f"cpu baseline on {threads} threads")
+        cpu = cpu_baseline(threads, args.cpu_steps)
+        log(f"cpu baseline {cpu['value']:.4f} GDOF-updates/s ({cpu['kind']})")
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 4),
+        "unit": "GDOF-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(el / args.steps * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded sine-mode initial field, constant velocity stored per node)",
+        "config": {"workload": args.workload, "NeX": ne[0], "NeY": ne[1], "NeZ": ne[2],
+                   "PolyOrder": p, "velocity": list(vel), "dt": dt, "variant": args.variant,
+                   "DOF": dof, "dof_updates_per_step": 3 * dof},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic_from_profiles(args.workload, args.variant),
+                     "kernel": "stage_kernel", "kernel_us": round(ms_stage * 1e3, 2),
+                     "algorithmic_bytes_per_launch": bytes_launch},
+        "cpu_baseline": cpu,
+        "checks": {"finite": finite, "mass_drift_rel": drift},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

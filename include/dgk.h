@@ -1,0 +1,176 @@
+/*
+ * dgk.h -- C-ABI of the MI355X (gfx950) DG advection operator library.
+ *
+ * The library replaces the operator-kernel layer of SCALE-DG's 3D advection
+ * extraction (Masahiro-Ren/scale-dg-kernel-extraction) with hand-written HIP
+ * kernels.  A Fortran host binds these entry points with iso_c_binding and
+ * selects them with DGOptrKernel_OptType = "HIP".
+ *
+ * Conventions (those of the reference, mod_advect3d_eq.f90:41-66):
+ *   - arrays are Fortran column-major and passed as plain pointers;
+ *   - fields are q(Np,NeA) with NeA = Ne + Nhalo; the halo slots
+ *     q(Np*Ne+1 : Np*Ne+NhaloNode) are filled from halo_src_map;
+ *   - VMapM/VMapP(NfpTot,Ne) and halo_src_map(NhaloNode) hold 1-based int32
+ *     flat indices into q(Np*NeA);
+ *   - real numbers are IEEE double (RP = real64, mod_common.f90:23).
+ * Host pointers unless a function says "device".  Every function returns
+ * DGK_OK (0) or a DGK_ERR_* code; dgk_last_error() describes the last failure
+ * of the calling thread.  Calls on one solver must come from one host thread.
+ */
+#ifndef DGK_H
+#define DGK_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DGK_OK 0
+#define DGK_ERR_ARG 1        /* invalid argument or size                  */
+#define DGK_ERR_OPTTYPE 2    /* unknown DGOptrKernel_OptType              */
+#define DGK_ERR_ORDER 3      /* polynomial order outside 1..15            */
+#define DGK_ERR_IO 4         /* operator data file missing or malformed   */
+#define DGK_ERR_HIP 5        /* HIP runtime error                         */
+#define DGK_ERR_NOMEM 6      /* device allocation failed                  */
+#define DGK_ERR_STATE 7      /* call not valid in the solver's state      */
+
+/* Operator evaluation types.  GENERAL and OPT1 are the reference's
+ * (mod_dg_optr_kernel.f90:39-42); HIP is the device path added here.  The
+ * summation order of a device run is GENERAL's or OPT1's. */
+#define DGK_TYPE_GENERAL 1
+#define DGK_TYPE_OPT1 2
+#define DGK_TYPE_HIP 3
+
+/* Solver flags (dgk_create / dgk_create_structured). */
+#define DGK_FLAG_FMA 0x1           /* fused multiply-add contractions (not
+                                      bit-identical to the reference)       */
+#define DGK_FLAG_FORCE_TABLES 0x2  /* always use the per-node VMapM/VMapP,
+                                      normal, Fscale and Escale tables       */
+
+/* Fields addressable through dgk_get_field / dgk_field_ptr. */
+#define DGK_FIELD_Q 0
+#define DGK_FIELD_Q0 1
+#define DGK_FIELD_DQDT 2
+#define DGK_FIELD_U 3
+#define DGK_FIELD_V 4
+#define DGK_FIELD_W 5
+
+typedef struct dgk_solver dgk_solver;
+
+/* ---- status ------------------------------------------------------------ */
+const char *dgk_last_error(void);
+int dgk_version(void);
+
+/* ---- selector ---------------------------------------------------------- */
+/* Maps "GENERAL" -> 1, "OPT1" -> 2, "HIP" -> 3 (trailing blanks ignored).
+ * Replaces dg_optr_kernel_setup (mod_dg_optr_kernel.f90:45-58), which only
+ * prints on an unknown name; here an unknown name returns DGK_ERR_OPTTYPE. */
+int dgk_optr_kernel_setup(const char *evaltype, int *type_id_out);
+
+/* ---- element and mesh producers (host) -------------------------------- */
+/* Reads operator_data/p<p>.dat: x1D(Nq), D1D(Nq,Nq), Lift_mat(Nq,Nq,Nq,6).
+ * Replaces element_init's read (mod_mesh.f90:253-263). */
+int dgk_read_operator(const char *dir, int p, double *x1D, double *D1D,
+                      double *Lift_mat);
+
+/* sizes[7] = {Nq, Np, Nfp, NfpTot, Ne, NeA, NhaloNode}
+ * (mod_mesh.f90:96-100, 238-241). */
+int dgk_mesh_sizes(int NeX, int NeY, int NeZ, int p, int32_t *sizes);
+
+/* Builds the periodic structured hexahedral mesh tables in the reference's
+ * numbering (mod_mesh.f90:74-193, 278-397): VMapM, VMapP(NfpTot,Ne),
+ * normal_fn(NfpTot,Ne,3), Escale(Np,Ne,3), Fscale(NfpTot,Ne),
+ * pos_en(Np,Ne,3), halo_src_map(NhaloNode).  Any output may be NULL. */
+int dgk_mesh_setup(int NeX, int NeY, int NeZ, int p, double domx, double domy,
+                   double domz, const double *x1D, int32_t *VMapM,
+                   int32_t *VMapP, double *normal_fn, double *Escale,
+                   double *Fscale, double *pos_en, int32_t *halo_src_map);
+
+/* ---- device-resident solver ------------------------------------------- */
+/* Creates a solver from the reference's mesh-level arrays (the argument list
+ * of advect3d_eq_cal_tend, mod_advect3d_eq.f90:41-66, plus the halo map of
+ * update_halo, mod_mesh.f90:209-221).  The tables are verified on the host;
+ * when they describe a periodic structured mesh with axis-aligned uniform
+ * geometry they are compressed to a 6-neighbour table and per-face/per-axis
+ * constants, otherwise the per-node tables are kept on the device.
+ * type_id: DGK_TYPE_GENERAL or DGK_TYPE_OPT1 (summation order); DGK_TYPE_HIP
+ * means OPT1's order. */
+int dgk_create(dgk_solver **out, int Nq, int Ne, int NeA, int NfpTot,
+               const double *D1D, const double *Lift_mat,
+               const int32_t *VMapM, const int32_t *VMapP,
+               const double *normal_fn, const double *Escale,
+               const double *Fscale, const int32_t *halo_src_map,
+               int NhaloNode, int type_id, int flags);
+
+/* Same solver for the periodic NeX x NeY x NeZ mesh on [0,domx]x[0,domy]x
+ * [0,domz] built directly in compressed form (no per-node tables). */
+int dgk_create_structured(dgk_solver **out, int NeX, int NeY, int NeZ, int p,
+                          double domx, double domy, double domz,
+                          const double *D1D, const double *Lift_mat,
+                          int type_id, int flags);
+
+int dgk_destroy(dgk_solver *s);
+
+/* Uploads q,u,v,w (Np,NeA) (owned parts used; halos are refilled on the
+ * device, as main.f90:159-162 does after the initial condition). */
+int dgk_set_fields(dgk_solver *s, const double *q, const double *u,
+                   const double *v, const double *w);
+/* Downloads one field (Np,NeA).  For q the halo slots hold the values of
+ * the current owned q. */
+int dgk_get_field(dgk_solver *s, int which, double *out);
+/* Device address of a field (Np*NeA doubles) for zero-copy callers. */
+int dgk_field_ptr(dgk_solver *s, int which, double **dptr);
+
+/* dqdt = advect3d_eq_cal_tend(q,u,v,w,...) on the device
+ * (mod_advect3d_eq.f90:41-85), halos of q refreshed first. */
+int dgk_cal_tend(dgk_solver *s);
+/* One SSP-RK3 stage (stage = 1..3): update_halo(q), tendency and
+ * q = rk_a(stage) q0 + rk_b(stage) (q + dt dqdt) fused in one launch; stage
+ * 1 also stores q0 <- q (main.f90:53-73). */
+int dgk_rk_stage(dgk_solver *s, int stage, double dt);
+/* One full time step (three stages) replayed from a captured hipGraph. */
+int dgk_step(dgk_solver *s, double dt);
+/* nstep time steps; if minmax is non-NULL it receives (min,max) of the owned
+ * q after every step, 2*nstep doubles (the main.f90:76-79 diagnostics). */
+int dgk_run(dgk_solver *s, double dt, int nstep, double *minmax);
+int dgk_synchronize(dgk_solver *s);
+/* what: 0 = connectivity mode (1 compressed, 0 tables), 1 = Ne, 2 = Np,
+ * 3 = NeA, 4 = grid size of the stage kernel, 5 = block size. */
+int dgk_query(dgk_solver *s, int what, int64_t *value);
+/* Times `nstage` consecutive stage launches with HIP events on the solver's
+ * stream; returns the mean milliseconds per stage launch. */
+int dgk_time_stages(dgk_solver *s, double dt, int nstage, double *ms_per_stage);
+
+/* ---- stateless drop-in of the mesh-level tendency ----------------------- */
+/* Same arguments and semantics as advect3d_eq_cal_tend
+ * (mod_advect3d_eq.f90:41-45); host arrays; q,u,v,w must carry their halos.
+ * The tendency is evaluated on the device in OPT1's summation order unless
+ * type_id is DGK_TYPE_GENERAL. */
+int dgk_advect3d_eq_cal_tend(double *dqdt, const double *q, const double *u,
+                             const double *v, const double *w,
+                             const double *D1D, const double *D1D_tr,
+                             const double *Lift_mat, const int32_t *VMapM,
+                             const int32_t *VMapP, const double *normal_fn,
+                             const double *Escale, const double *Fscale,
+                             int Nq, int Np, int NfpTot, int Ne, int NeA,
+                             int type_id);
+
+/* ---- batched element operators (device arrays) ------------------------- */
+/* tensorprod_divlike_dirXYZ (mod_dg_optr_kernel.f90:62-136) over nelem
+ * element blocks: in_x/in_y/in_z/out_* are (Np,nelem), Mat/Mat_tr (Nq,Nq).
+ * type_id selects GENERAL or OPT1 summation.  stream: hipStream_t or NULL. */
+int dgk_divlike_dirXYZ_batched(int Nq, int nelem, int type_id,
+                               const double *Mat, const double *Mat_tr,
+                               const double *in_x, const double *in_y,
+                               const double *in_z, double *out_x,
+                               double *out_y, double *out_z, void *stream);
+/* tensorprod_Lift_hexahedral (mod_dg_optr_kernel.f90:139-190) over nelem
+ * blocks: in (NfpTot,nelem), out (Np,nelem), Lift (Nq,Nq,Nq,6). */
+int dgk_lift_hexahedral_batched(int Nq, int nelem, const double *Lift,
+                                const double *in, double *out, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DGK_H */

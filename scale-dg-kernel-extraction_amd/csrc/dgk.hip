@@ -1,0 +1,676 @@
+// C-ABI of the device-resident DG advection solver (include/dgk.h): device
+// state, table verification/compression, launch dispatch over Nq and the
+// summation order, halo gathers and the hipGraph-captured time step.
+#include "dgk_internal.hpp"
+#include "dgk_kernels.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+using namespace dgk;
+
+#define HIPOK(expr)                                                            \
+  do {                                                                         \
+    hipError_t e_ = (expr);                                                    \
+    if (e_ != hipSuccess)                                                      \
+      return fail(DGK_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+namespace {
+
+using StageFn = void (*)(const StageParams);
+
+struct StageKernel {
+  StageFn fn = nullptr;
+  int nt = 0;
+  size_t lds = 0;
+};
+
+template <int NQ, int ORDER, bool FMA, bool STRUCT>
+StageKernel make_stage() {
+  StageKernel k;
+  k.fn = stage_kernel<NQ, ORDER, FMA, STRUCT>;
+  k.nt = StageShape<NQ>::NT;
+  k.lds = StageShape<NQ>::LDS;
+  return k;
+}
+
+template <int NQ>
+StageKernel pick_stage_nq(int order, bool fma, bool structured) {
+  const bool gen = order == kGeneral;
+  if (structured) {
+    if (gen) return fma ? make_stage<NQ, kGeneral, true, true>() : make_stage<NQ, kGeneral, false, true>();
+    return fma ? make_stage<NQ, kOpt1, true, true>() : make_stage<NQ, kOpt1, false, true>();
+  }
+  if (gen) return fma ? make_stage<NQ, kGeneral, true, false>() : make_stage<NQ, kGeneral, false, false>();
+  return fma ? make_stage<NQ, kOpt1, true, false>() : make_stage<NQ, kOpt1, false, false>();
+}
+
+StageKernel pick_stage(int Nq, int order, bool fma, bool structured) {
+  switch (Nq) {
+  case 2: return pick_stage_nq<2>(order, fma, structured);
+  case 3: return pick_stage_nq<3>(order, fma, structured);
+  case 4: return pick_stage_nq<4>(order, fma, structured);
+  case 5: return pick_stage_nq<5>(order, fma, structured);
+  case 6: return pick_stage_nq<6>(order, fma, structured);
+  case 7: return pick_stage_nq<7>(order, fma, structured);
+  case 8: return pick_stage_nq<8>(order, fma, structured);
+  case 9: return pick_stage_nq<9>(order, fma, structured);
+  case 10: return pick_stage_nq<10>(order, fma, structured);
+  case 11: return pick_stage_nq<11>(order, fma, structured);
+  case 12: return pick_stage_nq<12>(order, fma, structured);
+  case 13: return pick_stage_nq<13>(order, fma, structured);
+  case 14: return pick_stage_nq<14>(order, fma, structured);
+  case 15: return pick_stage_nq<15>(order, fma, structured);
+  case 16: return pick_stage_nq<16>(order, fma, structured);
+  default: return StageKernel{};
+  }
+}
+
+int order_of(int typeid_) {
+  if (typeid_ == DGK_TYPE_GENERAL) return kGeneral;
+  if (typeid_ == DGK_TYPE_OPT1 || typeid_ == DGK_TYPE_HIP) return kOpt1;
+  return -1;
+}
+
+const double kRkA[3] = {0.0, 0.75, 1.0 / 3.0};  // mod_common.f90:29
+const double kRkB[3] = {1.0, 0.25, 2.0 / 3.0};  // mod_common.f90:30
+
+template <class T>
+int dalloc(T **p, size_t n) {
+  *p = nullptr;
+  if (n == 0) return DGK_OK;
+  hipError_t e = hipMalloc((void **)p, n * sizeof(T));
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(DGK_ERR_NOMEM, "hipMalloc of %zu bytes failed: %s", n * sizeof(T),
+                hipGetErrorString(e));
+  }
+  return DGK_OK;
+}
+
+}  // namespace
+
+struct dgk_solver {
+  int Nq = 0, Np = 0, Nfp = 0, NfpTot = 0;
+  long long Ne = 0, NeA = 0;
+  int NhaloNode = 0;
+  int order = kOpt1;
+  bool fma = false;
+  bool structured = false;
+  double *q[2] = {nullptr, nullptr};
+  double *q0 = nullptr, *u = nullptr, *v = nullptr, *w = nullptr, *dqdt = nullptr;
+  double *D = nullptr, *Lift = nullptr;
+  int32_t *nbr = nullptr, *vmapM = nullptr, *vmapP = nullptr, *halo = nullptr;
+  double *nrm = nullptr, *fscale = nullptr, *escale = nullptr;
+  double fs[6] = {0, 0, 0, 0, 0, 0}, es[3] = {0, 0, 0};
+  double *part = nullptr, *mm = nullptr;
+  int mm_cap = 0;
+  int cur = 0;
+  hipStream_t stream = nullptr;
+  StageKernel sk;
+  int grid = 0;
+  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  double graph_dt = 0.0;
+  bool fields_set = false;
+};
+
+namespace {
+
+int configure_launch(dgk_solver *s) {
+  s->sk = pick_stage(s->Nq, s->order, s->fma, s->structured);
+  if (!s->sk.fn) return fail(DGK_ERR_ORDER, "Nq %d not supported", s->Nq);
+  HIPOK(hipFuncSetAttribute((const void *)s->sk.fn,
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)s->sk.lds));
+  int dev = 0, ncu = 0, per_cu = 0;
+  HIPOK(hipGetDevice(&dev));
+  HIPOK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  HIPOK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)s->sk.fn,
+                                                     s->sk.nt, s->sk.lds));
+  if (per_cu < 1) per_cu = 1;
+  const long long np = s->Np;
+  const long long e_per_group = np >= 256 ? 1 : 256 / np;
+  const long long ngroups = (s->Ne + e_per_group - 1) / e_per_group;
+  long long grid = (long long)per_cu * ncu;
+  const long long need = (ngroups + kNumXcd - 1) / kNumXcd * kNumXcd;
+  if (grid > need) grid = need;
+  grid = std::max<long long>(kNumXcd, grid / kNumXcd * kNumXcd);
+  s->grid = (int)grid;
+  return DGK_OK;
+}
+
+int alloc_fields(dgk_solver *s) {
+  const size_t nf = (size_t)s->Np * (size_t)s->NeA;
+  int rc;
+  if ((rc = dalloc(&s->q[0], nf)) || (rc = dalloc(&s->q[1], nf)) || (rc = dalloc(&s->q0, nf)) ||
+      (rc = dalloc(&s->u, nf)) || (rc = dalloc(&s->v, nf)) || (rc = dalloc(&s->w, nf)) ||
+      (rc = dalloc(&s->dqdt, nf)))
+    return rc;
+  HIPOK(hipMemsetAsync(s->q0, 0, nf * sizeof(double), s->stream));
+  HIPOK(hipMemsetAsync(s->dqdt, 0, nf * sizeof(double), s->stream));
+  HIPOK(hipMemsetAsync(s->q[1], 0, nf * sizeof(double), s->stream));
+  if ((rc = dalloc(&s->part, 2 * 1024))) return rc;
+  return DGK_OK;
+}
+
+int launch_halo(dgk_solver *s, double *f) {
+  if (s->NhaloNode == 0) return DGK_OK;
+  const int nb = (s->NhaloNode + 255) / 256;
+  hipLaunchKernelGGL(halo_gather_kernel, dim3(nb), dim3(256), 0, s->stream, f, s->halo,
+                     s->NhaloNode, (long long)s->Np * s->Ne);
+  HIPOK(hipGetLastError());
+  return DGK_OK;
+}
+
+StageParams make_params(dgk_solver *s, int mode, double a, double b, double dt) {
+  StageParams P{};
+  P.qin = s->q[s->cur];
+  P.qout = s->q[s->cur ^ 1];
+  P.q0 = s->q0;
+  P.u = s->u;
+  P.v = s->v;
+  P.w = s->w;
+  P.dqdt = s->dqdt;
+  P.D = s->D;
+  P.Lift = s->Lift;
+  P.nbr = s->nbr;
+  P.vmapM = s->vmapM;
+  P.vmapP = s->vmapP;
+  P.nrm = s->nrm;
+  P.fscale = s->fscale;
+  P.escale = s->escale;
+  for (int f = 0; f < 6; ++f) P.fs[f] = s->fs[f];
+  for (int d = 0; d < 3; ++d) P.es[d] = s->es[d];
+  P.Ne = s->Ne;
+  P.a = a;
+  P.b = b;
+  P.dt = dt;
+  P.mode = mode;
+  return P;
+}
+
+int launch_stage(dgk_solver *s, int mode, double a, double b, double dt) {
+  if (!s->structured) {
+    int rc = launch_halo(s, s->q[s->cur]);
+    if (rc) return rc;
+  }
+  StageParams P = make_params(s, mode, a, b, dt);
+  hipLaunchKernelGGL(s->sk.fn, dim3(s->grid), dim3(s->sk.nt), s->sk.lds, s->stream, P);
+  HIPOK(hipGetLastError());
+  if (mode != kModeTend) s->cur ^= 1;
+  return DGK_OK;
+}
+
+void drop_graphs(dgk_solver *s) {
+  for (auto &g : s->graph)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+}
+
+// Verifies that the reference tables describe a periodic structured mesh with
+// axis-aligned unit normals and uniform Fscale/Escale, and if so fills the
+// compressed neighbour table and constants.  Returns false otherwise.
+bool compress_tables(int Nq, long long Ne, const int32_t *VMapM, const int32_t *VMapP,
+                     const double *nrm, const double *Escale, const double *Fscale,
+                     const int32_t *halo, int NhaloNode, std::vector<int32_t> &nbr,
+                     double fs[6], double es[3]) {
+  const int Np = Nq * Nq * Nq, Nfp = Nq * Nq, NfpTot = 6 * Nfp;
+  const long long NfNe = (long long)NfpTot * Ne, NpNe = (long long)Np * Ne;
+  const long long owned = (long long)Np * Ne;
+  std::vector<int> fnode(6 * Nfp);
+  for (int f = 0; f < 6; ++f)
+    for (int fp = 0; fp < Nfp; ++fp) fnode[f * Nfp + fp] = face_node_host(Nq, f, fp % Nq, fp / Nq);
+  nbr.assign(6 * Ne, 0);
+  for (int f = 0; f < 6; ++f) fs[f] = Fscale[f * Nfp];
+  for (int d = 0; d < 3; ++d) es[d] = Escale[d * NpNe];
+  for (long long e = 0; e < Ne; ++e) {
+    for (int d = 0; d < 3; ++d)
+      for (int n = 0; n < Np; ++n)
+        if (Escale[n + e * Np + d * NpNe] != es[d]) return false;
+    for (int f = 0; f < 6; ++f) {
+      const int opp = kFace[f].opp;
+      long long eP = -1;
+      for (int fp = 0; fp < Nfp; ++fp) {
+        const long long t = f * Nfp + fp + e * NfpTot;
+        for (int d = 0; d < 3; ++d) {
+          const double want = d == kFace[f].axis ? (double)kFace[f].sign : 0.0;
+          if (nrm[t + d * NfNe] != want) return false;
+        }
+        if (Fscale[t] != fs[f]) return false;
+        if ((long long)VMapM[t] != fnode[f * Nfp + fp] + 1 + e * Np) return false;
+        long long iP = (long long)VMapP[t] - 1;
+        if (iP >= owned) {
+          const long long ib = iP - owned;
+          if (ib >= NhaloNode) return false;
+          iP = (long long)halo[ib] - 1;
+        }
+        if (iP < 0 || iP >= owned) return false;
+        const long long cand = iP / Np;
+        if (iP - cand * Np != fnode[opp * Nfp + fp]) return false;
+        if (eP < 0) eP = cand;
+        else if (cand != eP) return false;
+      }
+      nbr[e * 6 + f] = (int32_t)eP;
+    }
+  }
+  return true;
+}
+
+int finish_create(dgk_solver *s, const double *D1D, const double *Lift) {
+  int rc;
+  const size_t nq2 = (size_t)s->Nq * s->Nq, nlift = (size_t)6 * s->Np;
+  if ((rc = dalloc(&s->D, nq2)) || (rc = dalloc(&s->Lift, nlift))) return rc;
+  HIPOK(hipMemcpy(s->D, D1D, nq2 * sizeof(double), hipMemcpyHostToDevice));
+  HIPOK(hipMemcpy(s->Lift, Lift, nlift * sizeof(double), hipMemcpyHostToDevice));
+  if ((rc = alloc_fields(s))) return rc;
+  if ((rc = configure_launch(s))) return rc;
+  HIPOK(hipStreamSynchronize(s->stream));
+  return DGK_OK;
+}
+
+template <class T>
+int upload(T **dst, const T *src, size_t n, hipStream_t st) {
+  int rc = dalloc(dst, n);
+  if (rc) return rc;
+  if (n) HIPOK(hipMemcpyAsync(*dst, src, n * sizeof(T), hipMemcpyHostToDevice, st));
+  HIPOK(hipStreamSynchronize(st));
+  return DGK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dgk_destroy(dgk_solver *s) {
+  if (!s) return DGK_OK;
+  drop_graphs(s);
+  void *bufs[] = {s->q[0], s->q[1], s->q0, s->u, s->v, s->w, s->dqdt, s->D, s->Lift,
+                  s->nbr, s->vmapM, s->vmapP, s->halo, s->nrm, s->fscale, s->escale,
+                  s->part, s->mm};
+  for (void *b : bufs)
+    if (b) (void)hipFree(b);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+  return DGK_OK;
+}
+
+int dgk_create(dgk_solver **out, int Nq, int Ne, int NeA, int NfpTot, const double *D1D,
+               const double *Lift_mat, const int32_t *VMapM, const int32_t *VMapP,
+               const double *normal_fn, const double *Escale, const double *Fscale,
+               const int32_t *halo_src_map, int NhaloNode, int typeid_, int flags) {
+  clear_error();
+  if (!out) return fail(DGK_ERR_ARG, "null solver pointer");
+  *out = nullptr;
+  if (Nq < 2 || Nq > 16) return fail(DGK_ERR_ORDER, "PolyOrder %d outside 1..15", Nq - 1);
+  if (NfpTot != 6 * Nq * Nq || Ne < 1 || NeA < Ne || NhaloNode < 0)
+    return fail(DGK_ERR_ARG, "inconsistent sizes Nq=%d NfpTot=%d Ne=%d NeA=%d", Nq, NfpTot, Ne,
+                NeA);
+  const long long Np = (long long)Nq * Nq * Nq;
+  if (NhaloNode > Np * (NeA - Ne)) return fail(DGK_ERR_ARG, "NhaloNode exceeds the halo slack");
+  if (!D1D || !Lift_mat || !VMapM || !VMapP || !normal_fn || !Escale || !Fscale ||
+      (NhaloNode && !halo_src_map))
+    return fail(DGK_ERR_ARG, "null table");
+  const int order = order_of(typeid_);
+  if (order < 0) return fail(DGK_ERR_OPTTYPE, "typeid %d is not GENERAL, OPT1 or HIP", typeid_);
+  // Validate index ranges before anything reaches the device.
+  const long long NfNe = (long long)NfpTot * Ne, total = Np * NeA;
+  for (long long t = 0; t < NfNe; ++t)
+    if (VMapM[t] < 1 || VMapM[t] > total || VMapP[t] < 1 || VMapP[t] > total)
+      return fail(DGK_ERR_ARG, "VMapM/VMapP entry %lld out of 1..Np*NeA", t + 1);
+  for (int ib = 0; ib < NhaloNode; ++ib)
+    if (halo_src_map[ib] < 1 || halo_src_map[ib] > Np * Ne)
+      return fail(DGK_ERR_ARG, "halo_src_map entry %d out of the owned range", ib + 1);
+
+  dgk_solver *s = new dgk_solver;
+  s->Nq = Nq;
+  s->Np = (int)Np;
+  s->Nfp = Nq * Nq;
+  s->NfpTot = NfpTot;
+  s->Ne = Ne;
+  s->NeA = NeA;
+  s->NhaloNode = NhaloNode;
+  s->order = order;
+  s->fma = (flags & DGK_FLAG_FMA) != 0;
+  int rc = DGK_OK;
+  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete s;
+    return fail(DGK_ERR_HIP, "hipStreamCreate failed");
+  }
+  std::vector<int32_t> nbr;
+  s->structured = !(flags & DGK_FLAG_FORCE_TABLES) &&
+                  compress_tables(Nq, Ne, VMapM, VMapP, normal_fn, Escale, Fscale, halo_src_map,
+                                  NhaloNode, nbr, s->fs, s->es);
+  if (s->structured) {
+    rc = upload(&s->nbr, nbr.data(), nbr.size(), s->stream);
+  } else {
+    if (!rc) rc = upload(&s->vmapM, VMapM, (size_t)NfNe, s->stream);
+    if (!rc) rc = upload(&s->vmapP, VMapP, (size_t)NfNe, s->stream);
+    if (!rc) rc = upload(&s->nrm, normal_fn, (size_t)(3 * NfNe), s->stream);
+    if (!rc) rc = upload(&s->fscale, Fscale, (size_t)NfNe, s->stream);
+    if (!rc) rc = upload(&s->escale, Escale, (size_t)(3 * Np * Ne), s->stream);
+  }
+  if (!rc) rc = upload(&s->halo, halo_src_map, (size_t)NhaloNode, s->stream);
+  if (!rc) rc = finish_create(s, D1D, Lift_mat);
+  if (rc) {
+    dgk_destroy(s);
+    return rc;
+  }
+  *out = s;
+  return DGK_OK;
+}
+
+int dgk_create_structured(dgk_solver **out, int NeX, int NeY, int NeZ, int p, double domx,
+                          double domy, double domz, const double *D1D, const double *Lift_mat,
+                          int typeid_, int flags) {
+  clear_error();
+  if (!out) return fail(DGK_ERR_ARG, "null solver pointer");
+  *out = nullptr;
+  int32_t sz[7];
+  int rc = dgk_mesh_sizes(NeX, NeY, NeZ, p, sz);
+  if (rc) return rc;
+  if (flags & DGK_FLAG_FORCE_TABLES)
+    return fail(DGK_ERR_ARG, "DGK_FLAG_FORCE_TABLES needs dgk_create with tables");
+  if (!D1D || !Lift_mat) return fail(DGK_ERR_ARG, "null operator");
+  const int order = order_of(typeid_);
+  if (order < 0) return fail(DGK_ERR_OPTTYPE, "typeid %d is not GENERAL, OPT1 or HIP", typeid_);
+  dgk_solver *s = new dgk_solver;
+  s->Nq = sz[0];
+  s->Np = sz[1];
+  s->Nfp = sz[2];
+  s->NfpTot = sz[3];
+  s->Ne = sz[4];
+  s->NeA = sz[5];
+  s->NhaloNode = sz[6];
+  s->order = order;
+  s->fma = (flags & DGK_FLAG_FMA) != 0;
+  s->structured = true;
+  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete s;
+    return fail(DGK_ERR_HIP, "hipStreamCreate failed");
+  }
+  const double dl[3] = {domx / (double)NeX, domy / (double)NeY, domz / (double)NeZ};
+  for (int f = 0; f < 6; ++f) s->fs[f] = 2.0 / dl[kFace[f].axis];  // mod_mesh.f90:366-394
+  for (int d = 0; d < 3; ++d) s->es[d] = 2.0 / dl[d];               // mod_mesh.f90:147-149
+  std::vector<int32_t> nbr(6 * (size_t)s->Ne);
+  for (int k = 0; k < NeZ; ++k)
+    for (int j = 0; j < NeY; ++j)
+      for (int i = 0; i < NeX; ++i) {
+        const long long e = i + (long long)j * NeX + (long long)k * NeX * NeY;
+        for (int f = 0; f < 6; ++f) {
+          bool wrapped;
+          nbr[e * 6 + f] = (int32_t)neighbour_elem(f, i, j, k, NeX, NeY, NeZ, &wrapped);
+        }
+      }
+  std::vector<int32_t> halo(s->NhaloNode);
+  rc = dgk_mesh_setup(NeX, NeY, NeZ, p, domx, domy, domz, nullptr, nullptr, nullptr, nullptr,
+                      nullptr, nullptr, nullptr, halo.data());
+  if (!rc) rc = upload(&s->nbr, nbr.data(), nbr.size(), s->stream);
+  if (!rc) rc = upload(&s->halo, halo.data(), halo.size(), s->stream);
+  if (!rc) rc = finish_create(s, D1D, Lift_mat);
+  if (rc) {
+    dgk_destroy(s);
+    return rc;
+  }
+  *out = s;
+  return DGK_OK;
+}
+
+int dgk_set_fields(dgk_solver *s, const double *q, const double *u, const double *v,
+                   const double *w) {
+  if (!s || !q || !u || !v || !w) return fail(DGK_ERR_ARG, "null argument");
+  const size_t nb = (size_t)s->Np * s->NeA * sizeof(double);
+  s->cur = 0;
+  HIPOK(hipMemcpyAsync(s->q[0], q, nb, hipMemcpyHostToDevice, s->stream));
+  HIPOK(hipMemcpyAsync(s->u, u, nb, hipMemcpyHostToDevice, s->stream));
+  HIPOK(hipMemcpyAsync(s->v, v, nb, hipMemcpyHostToDevice, s->stream));
+  HIPOK(hipMemcpyAsync(s->w, w, nb, hipMemcpyHostToDevice, s->stream));
+  int rc;
+  for (double *f : {s->q[0], s->u, s->v, s->w})
+    if ((rc = launch_halo(s, f))) return rc;
+  HIPOK(hipStreamSynchronize(s->stream));
+  s->fields_set = true;
+  return DGK_OK;
+}
+
+int dgk_field_ptr(dgk_solver *s, int which, double **dptr) {
+  if (!s || !dptr) return fail(DGK_ERR_ARG, "null argument");
+  switch (which) {
+  case DGK_FIELD_Q: *dptr = s->q[s->cur]; break;
+  case DGK_FIELD_Q0: *dptr = s->q0; break;
+  case DGK_FIELD_DQDT: *dptr = s->dqdt; break;
+  case DGK_FIELD_U: *dptr = s->u; break;
+  case DGK_FIELD_V: *dptr = s->v; break;
+  case DGK_FIELD_W: *dptr = s->w; break;
+  default: return fail(DGK_ERR_ARG, "unknown field %d", which);
+  }
+  return DGK_OK;
+}
+
+int dgk_get_field(dgk_solver *s, int which, double *out) {
+  double *src = nullptr;
+  int rc = dgk_field_ptr(s, which, &src);
+  if (rc) return rc;
+  if (!out) return fail(DGK_ERR_ARG, "null output");
+  if (which == DGK_FIELD_Q && (rc = launch_halo(s, src))) return rc;
+  HIPOK(hipMemcpyAsync(out, src, (size_t)s->Np * s->NeA * sizeof(double),
+                       hipMemcpyDeviceToHost, s->stream));
+  HIPOK(hipStreamSynchronize(s->stream));
+  return DGK_OK;
+}
+
+int dgk_cal_tend(dgk_solver *s) {
+  if (!s) return fail(DGK_ERR_ARG, "null solver");
+  if (!s->fields_set) return fail(DGK_ERR_STATE, "dgk_set_fields has not been called");
+  return launch_stage(s, kModeTend, 0.0, 0.0, 0.0);
+}
+
+int dgk_rk_stage(dgk_solver *s, int stage, double dt) {
+  if (!s) return fail(DGK_ERR_ARG, "null solver");
+  if (!s->fields_set) return fail(DGK_ERR_STATE, "dgk_set_fields has not been called");
+  if (stage < 1 || stage > 3) return fail(DGK_ERR_ARG, "stage %d outside 1..3", stage);
+  return launch_stage(s, stage == 1 ? kModeStage1 : kModeStageN, kRkA[stage - 1],
+                      kRkB[stage - 1], dt);
+}
+
+int dgk_step(dgk_solver *s, double dt) {
+  if (!s) return fail(DGK_ERR_ARG, "null solver");
+  if (!s->fields_set) return fail(DGK_ERR_STATE, "dgk_set_fields has not been called");
+  if (s->graph_dt != dt) drop_graphs(s);
+  // Three stages flip the ping-pong buffers an odd number of times, so two
+  // graphs exist: one per starting buffer.
+  const int c0 = s->cur;
+  if (!s->graph[c0]) {
+    hipGraph_t g;
+    HIPOK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    int rc = DGK_OK;
+    for (int st = 1; st <= 3 && !rc; ++st)
+      rc = launch_stage(s, st == 1 ? kModeStage1 : kModeStageN, kRkA[st - 1], kRkB[st - 1], dt);
+    hipError_t ce = hipStreamEndCapture(s->stream, &g);
+    s->cur = c0;
+    if (rc) return rc;
+    if (ce != hipSuccess) return fail(DGK_ERR_HIP, "graph capture: %s", hipGetErrorString(ce));
+    hipError_t ie = hipGraphInstantiate(&s->graph[c0], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ie != hipSuccess) return fail(DGK_ERR_HIP, "graph instantiate: %s", hipGetErrorString(ie));
+    s->graph_dt = dt;
+  }
+  HIPOK(hipGraphLaunch(s->graph[c0], s->stream));
+  s->cur = c0 ^ 1;
+  return DGK_OK;
+}
+
+int dgk_run(dgk_solver *s, double dt, int nstep, double *minmax) {
+  if (!s) return fail(DGK_ERR_ARG, "null solver");
+  if (nstep < 0) return fail(DGK_ERR_ARG, "negative nstep");
+  if (minmax && nstep > s->mm_cap) {
+    if (s->mm) (void)hipFree(s->mm);
+    s->mm = nullptr;
+    int rc = dalloc(&s->mm, 2 * (size_t)nstep);
+    if (rc) return rc;
+    s->mm_cap = nstep;
+  }
+  const long long owned = (long long)s->Np * s->Ne;
+  const int nb = (int)std::min<long long>(1024, (owned + 255) / 256);
+  for (int it = 0; it < nstep; ++it) {
+    int rc = dgk_step(s, dt);
+    if (rc) return rc;
+    if (minmax) {
+      hipLaunchKernelGGL(minmax_partial_kernel, dim3(nb), dim3(256), 0, s->stream,
+                         s->q[s->cur], owned, s->part);
+      hipLaunchKernelGGL(minmax_final_kernel, dim3(1), dim3(256), 0, s->stream, s->part, nb,
+                         s->mm + 2 * it);
+      HIPOK(hipGetLastError());
+    }
+  }
+  if (minmax && nstep)
+    HIPOK(hipMemcpyAsync(minmax, s->mm, 2 * (size_t)nstep * sizeof(double),
+                         hipMemcpyDeviceToHost, s->stream));
+  HIPOK(hipStreamSynchronize(s->stream));
+  return DGK_OK;
+}
+
+int dgk_synchronize(dgk_solver *s) {
+  if (!s) return fail(DGK_ERR_ARG, "null solver");
+  HIPOK(hipStreamSynchronize(s->stream));
+  return DGK_OK;
+}
+
+int dgk_query(dgk_solver *s, int what, int64_t *value) {
+  if (!s || !value) return fail(DGK_ERR_ARG, "null argument");
+  switch (what) {
+  case 0: *value = s->structured ? 1 : 0; break;
+  case 1: *value = s->Ne; break;
+  case 2: *value = s->Np; break;
+  case 3: *value = s->NeA; break;
+  case 4: *value = s->grid; break;
+  case 5: *value = s->sk.nt; break;
+  default: return fail(DGK_ERR_ARG, "unknown query %d", what);
+  }
+  return DGK_OK;
+}
+
+int dgk_time_stages(dgk_solver *s, double dt, int nstage, double *ms_per_stage) {
+  if (!s || !ms_per_stage || nstage < 1) return fail(DGK_ERR_ARG, "bad argument");
+  if (!s->fields_set) return fail(DGK_ERR_STATE, "dgk_set_fields has not been called");
+  hipEvent_t e0, e1;
+  HIPOK(hipEventCreate(&e0));
+  HIPOK(hipEventCreate(&e1));
+  HIPOK(hipEventRecord(e0, s->stream));
+  for (int it = 0; it < nstage; ++it) {
+    const int st = it % 3;
+    int rc = launch_stage(s, st == 0 ? kModeStage1 : kModeStageN, kRkA[st], kRkB[st], dt);
+    if (rc) return rc;
+  }
+  HIPOK(hipEventRecord(e1, s->stream));
+  HIPOK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIPOK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms_per_stage = ms / nstage;
+  return DGK_OK;
+}
+
+int dgk_advect3d_eq_cal_tend(double *dqdt, const double *q, const double *u, const double *v,
+                             const double *w, const double *D1D, const double *D1D_tr,
+                             const double *Lift_mat, const int32_t *VMapM,
+                             const int32_t *VMapP, const double *normal_fn,
+                             const double *Escale, const double *Fscale, int Nq, int Np,
+                             int NfpTot, int Ne, int NeA, int typeid_) {
+  clear_error();
+  if (!dqdt || !q || !u || !v || !w) return fail(DGK_ERR_ARG, "null field");
+  if (Np != Nq * Nq * Nq) return fail(DGK_ERR_ARG, "Np != Nq^3");
+  (void)D1D_tr;  // the transpose is implied by D1D (mod_mesh.f90:263)
+  dgk_solver *s = nullptr;
+  // The caller's q,u,v,w carry their halos already, so no halo map is needed:
+  // the table path gathers through VMapP directly.
+  int rc = dgk_create(&s, Nq, Ne, NeA, NfpTot, D1D, Lift_mat, VMapM, VMapP, normal_fn, Escale,
+                      Fscale, nullptr, 0, typeid_ == DGK_TYPE_GENERAL ? DGK_TYPE_GENERAL
+                                                                       : DGK_TYPE_OPT1,
+                      DGK_FLAG_FORCE_TABLES);
+  if (rc) return rc;
+  const size_t nb = (size_t)Np * NeA * sizeof(double);
+  s->cur = 0;
+  rc = DGK_OK;
+  if (hipMemcpy(s->q[0], q, nb, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(s->u, u, nb, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(s->v, v, nb, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(s->w, w, nb, hipMemcpyHostToDevice) != hipSuccess)
+    rc = fail(DGK_ERR_HIP, "field upload failed");
+  s->fields_set = true;
+  if (!rc) rc = launch_stage(s, kModeTend, 0.0, 0.0, 0.0);
+  if (!rc && hipStreamSynchronize(s->stream) != hipSuccess)
+    rc = fail(DGK_ERR_HIP, "tendency kernel failed");
+  // dqdt(:,1:Ne) only, as cal_dqdt writes (mod_advect3d_eq.f90:194)
+  if (!rc && hipMemcpy(dqdt, s->dqdt, (size_t)Np * Ne * sizeof(double), hipMemcpyDeviceToHost) !=
+                 hipSuccess)
+    rc = fail(DGK_ERR_HIP, "dqdt download failed");
+  dgk_destroy(s);
+  return rc;
+}
+
+int dgk_divlike_dirXYZ_batched(int Nq, int nelem, int typeid_, const double *Mat,
+                               const double *Mat_tr, const double *in_x, const double *in_y,
+                               const double *in_z, double *out_x, double *out_y, double *out_z,
+                               void *stream) {
+  clear_error();
+  if (Nq < 2 || Nq > 16) return fail(DGK_ERR_ORDER, "PolyOrder %d outside 1..15", Nq - 1);
+  if (nelem < 0) return fail(DGK_ERR_ARG, "negative nelem");
+  if (nelem == 0) return DGK_OK;
+  const int order = order_of(typeid_);
+  if (order < 0) return fail(DGK_ERR_OPTTYPE, "typeid %d", typeid_);
+  if (!Mat || !Mat_tr || !in_x || !in_y || !in_z || !out_x || !out_y || !out_z)
+    return fail(DGK_ERR_ARG, "null pointer");
+  const size_t lds = sizeof(double) * (3 * (size_t)Nq * Nq * Nq + 2 * (size_t)Nq * Nq);
+  hipStream_t st = (hipStream_t)stream;
+  using Fn = void (*)(const double *, const double *, const double *, const double *,
+                      const double *, double *, double *, double *);
+  Fn fn = nullptr;
+#define DGK_DIV_CASE(NQ)                                                                   \
+  case NQ:                                                                                 \
+    fn = order == kGeneral ? divlike_batched_kernel<NQ, kGeneral> : divlike_batched_kernel<NQ, kOpt1>; \
+    break;
+  switch (Nq) {
+    DGK_DIV_CASE(2) DGK_DIV_CASE(3) DGK_DIV_CASE(4) DGK_DIV_CASE(5) DGK_DIV_CASE(6)
+    DGK_DIV_CASE(7) DGK_DIV_CASE(8) DGK_DIV_CASE(9) DGK_DIV_CASE(10) DGK_DIV_CASE(11)
+    DGK_DIV_CASE(12) DGK_DIV_CASE(13) DGK_DIV_CASE(14) DGK_DIV_CASE(15) DGK_DIV_CASE(16)
+  }
+#undef DGK_DIV_CASE
+  HIPOK(hipFuncSetAttribute((const void *)fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)lds));
+  hipLaunchKernelGGL(fn, dim3(nelem), dim3(256), lds, st, Mat, Mat_tr, in_x, in_y, in_z, out_x,
+                     out_y, out_z);
+  HIPOK(hipGetLastError());
+  return DGK_OK;
+}
+
+int dgk_lift_hexahedral_batched(int Nq, int nelem, const double *Lift, const double *in,
+                                double *out, void *stream) {
+  clear_error();
+  if (Nq < 2 || Nq > 16) return fail(DGK_ERR_ORDER, "PolyOrder %d outside 1..15", Nq - 1);
+  if (nelem < 0) return fail(DGK_ERR_ARG, "negative nelem");
+  if (nelem == 0) return DGK_OK;
+  if (!Lift || !in || !out) return fail(DGK_ERR_ARG, "null pointer");
+  const size_t lds = sizeof(double) * 6 * (size_t)Nq * Nq;
+  hipStream_t st = (hipStream_t)stream;
+  using Fn = void (*)(const double *, const double *, double *);
+  Fn fn = nullptr;
+#define DGK_LIFT_CASE(NQ) \
+  case NQ: fn = lift_batched_kernel<NQ>; break;
+  switch (Nq) {
+    DGK_LIFT_CASE(2) DGK_LIFT_CASE(3) DGK_LIFT_CASE(4) DGK_LIFT_CASE(5) DGK_LIFT_CASE(6)
+    DGK_LIFT_CASE(7) DGK_LIFT_CASE(8) DGK_LIFT_CASE(9) DGK_LIFT_CASE(10) DGK_LIFT_CASE(11)
+    DGK_LIFT_CASE(12) DGK_LIFT_CASE(13) DGK_LIFT_CASE(14) DGK_LIFT_CASE(15) DGK_LIFT_CASE(16)
+  }
+#undef DGK_LIFT_CASE
+  hipLaunchKernelGGL(fn, dim3(nelem), dim3(256), lds, st, Lift, in, out);
+  HIPOK(hipGetLastError());
+  return DGK_OK;
+}
+
+}  // extern "C"

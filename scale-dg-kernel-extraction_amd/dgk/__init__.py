@@ -1,0 +1,272 @@
+"""ctypes binding of libdgk.so (include/dgk.h) for Python hosts and tests.
+
+Mirrors the reference's operator interface: ``optr_kernel_setup`` is
+dg_optr_kernel_setup (mod_dg_optr_kernel.f90:45-58), ``mesh_setup`` is the
+reference's mesh_setup (mod_mesh.f90:74-193), ``Solver.cal_tend`` is
+advect3d_eq_cal_tend (mod_advect3d_eq.f90:41-85) and ``Solver.step`` one
+SSP-RK3 step of main.f90:51-80.  Arrays follow the reference's Fortran
+layouts (numpy ``order="F"``).  The library is required: nothing here falls
+back to a CPU implementation.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+REPO = os.path.dirname(PKG)
+LIB_PATH = os.path.join(PKG, "lib", "libdgk.so")
+OPERATOR_DIR = os.path.join(REPO, "operator_data")
+
+OK = 0
+ERR_ARG, ERR_OPTTYPE, ERR_ORDER, ERR_IO, ERR_HIP, ERR_NOMEM, ERR_STATE = range(1, 8)
+TYPE_GENERAL, TYPE_OPT1, TYPE_HIP = 1, 2, 3
+FLAG_FMA, FLAG_FORCE_TABLES = 0x1, 0x2
+FIELD_Q, FIELD_Q0, FIELD_DQDT, FIELD_U, FIELD_V, FIELD_W = range(6)
+
+EXPORTS = (
+    "dgk_last_error", "dgk_version", "dgk_optr_kernel_setup", "dgk_read_operator",
+    "dgk_mesh_sizes", "dgk_mesh_setup", "dgk_create", "dgk_create_structured",
+    "dgk_destroy", "dgk_set_fields", "dgk_get_field", "dgk_field_ptr", "dgk_cal_tend",
+    "dgk_rk_stage", "dgk_step", "dgk_run", "dgk_synchronize", "dgk_query",
+    "dgk_time_stages", "dgk_advect3d_eq_cal_tend", "dgk_divlike_dirXYZ_batched",
+    "dgk_lift_hexahedral_batched",
+)
+
+
+class DGKError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"dgk error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """Loads libdgk.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with "
+                          "`make -C scale-dg-kernel-extraction_amd/csrc` or "
+                          "`python -c 'import __graft_entry__ as g; g.build()'`")
+    L = C.CDLL(LIB_PATH)
+    i, d, vp, ll = C.c_int, C.c_double, C.c_void_p, C.c_longlong
+    sig = {
+        "dgk_last_error": ([], C.c_char_p),
+        "dgk_version": ([], i),
+        "dgk_optr_kernel_setup": ([C.c_char_p, vp], i),
+        "dgk_read_operator": ([C.c_char_p, i, vp, vp, vp], i),
+        "dgk_mesh_sizes": ([i, i, i, i, vp], i),
+        "dgk_mesh_setup": ([i, i, i, i, d, d, d] + [vp] * 8, i),
+        "dgk_create": ([vp, i, i, i, i] + [vp] * 8 + [i, i, i], i),
+        "dgk_create_structured": ([vp, i, i, i, i, d, d, d, vp, vp, i, i], i),
+        "dgk_destroy": ([vp], i),
+        "dgk_set_fields": ([vp] * 5, i),
+        "dgk_get_field": ([vp, i, vp], i),
+        "dgk_field_ptr": ([vp, i, vp], i),
+        "dgk_cal_tend": ([vp], i),
+        "dgk_rk_stage": ([vp, i, d], i),
+        "dgk_step": ([vp, d], i),
+        "dgk_run": ([vp, d, i, vp], i),
+        "dgk_synchronize": ([vp], i),
+        "dgk_query": ([vp, i, vp], i),
+        "dgk_time_stages": ([vp, d, i, vp], i),
+        "dgk_advect3d_eq_cal_tend": ([vp] * 13 + [i] * 6, i),
+        "dgk_divlike_dirXYZ_batched": ([i, i, i] + [vp] * 9, i),
+        "dgk_lift_hexahedral_batched": ([i, i, vp, vp, vp, vp], i),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc != OK:
+        raise DGKError(rc, lib().dgk_last_error().decode(errors="replace"))
+
+
+def _p(a):
+    if a is None:
+        return None
+    if not (a.flags.f_contiguous or a.ndim <= 1):
+        raise ValueError("array must be Fortran-contiguous")
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def optr_kernel_setup(evaltype: str) -> int:
+    """DGOptrKernel_OptType name -> type id (GENERAL 1, OPT1 2, HIP 3)."""
+    t = C.c_int(0)
+    _check(lib().dgk_optr_kernel_setup(evaltype.encode(), C.byref(t)))
+    return t.value
+
+
+def read_operator(p: int, opdir: str = OPERATOR_DIR):
+    """x1D(Nq), D1D(Nq,Nq), Lift_mat(Nq,Nq,Nq,6) from operator_data/p<p>.dat."""
+    Nq = p + 1
+    x1D = np.zeros(Nq)
+    D = np.zeros((Nq, Nq), order="F")
+    L = np.zeros((Nq, Nq, Nq, 6), order="F")
+    _check(lib().dgk_read_operator(opdir.encode(), p, _p(x1D), _p(D), _p(L)))
+    return x1D, D, L
+
+
+def mesh_sizes(NeX, NeY, NeZ, p):
+    s = np.zeros(7, dtype=np.int32)
+    _check(lib().dgk_mesh_sizes(NeX, NeY, NeZ, p, _p(s)))
+    return dict(zip(("Nq", "Np", "Nfp", "NfpTot", "Ne", "NeA", "NhaloNode"),
+                    (int(v) for v in s)))
+
+
+def mesh_setup(NeX, NeY, NeZ, p, dom=(1.0, 1.0, 1.0), x1D=None, tables=True):
+    """Reference-layout mesh tables (mod_mesh.f90:74-193) as a dict."""
+    sz = mesh_sizes(NeX, NeY, NeZ, p)
+    Np, Nf, Ne = sz["Np"], sz["NfpTot"], sz["Ne"]
+    out = dict(sz)
+    if x1D is None:
+        x1D = read_operator(p)[0]
+    pos = np.zeros((Np, Ne, 3), order="F")
+    halo = np.zeros(sz["NhaloNode"], dtype=np.int32)
+    arrs = dict(VMapM=None, VMapP=None, normal_fn=None, Escale=None, Fscale=None)
+    if tables:
+        arrs = dict(VMapM=np.zeros((Nf, Ne), dtype=np.int32, order="F"),
+                    VMapP=np.zeros((Nf, Ne), dtype=np.int32, order="F"),
+                    normal_fn=np.zeros((Nf, Ne, 3), order="F"),
+                    Escale=np.zeros((Np, Ne, 3), order="F"),
+                    Fscale=np.zeros((Nf, Ne), order="F"))
+    _check(lib().dgk_mesh_setup(NeX, NeY, NeZ, p, float(dom[0]), float(dom[1]),
+                                float(dom[2]), _p(x1D), _p(arrs["VMapM"]),
+                                _p(arrs["VMapP"]), _p(arrs["normal_fn"]),
+                                _p(arrs["Escale"]), _p(arrs["Fscale"]), _p(pos),
+                                _p(halo)))
+    out.update(arrs)
+    out.update(pos_en=pos, halo_src_map=halo, x1D=x1D)
+    return out
+
+
+class Solver:
+    """Device-resident tendency/RK solver (dgk_create / dgk_create_structured)."""
+
+    def __init__(self, handle, Np, Ne, NeA):
+        self._h = handle
+        self.Np, self.Ne, self.NeA = Np, Ne, NeA
+
+    @classmethod
+    def from_tables(cls, Nq, Ne, NeA, D1D, Lift, VMapM, VMapP, normal_fn, Escale,
+                    Fscale, halo_src_map, typeid=TYPE_HIP, flags=0):
+        h = C.c_void_p()
+        D1D, Lift = np.asfortranarray(D1D), np.asfortranarray(Lift)
+        _check(lib().dgk_create(C.byref(h), Nq, Ne, NeA, 6 * Nq * Nq, _p(D1D), _p(Lift),
+                                _p(VMapM), _p(VMapP), _p(normal_fn), _p(Escale),
+                                _p(Fscale), _p(halo_src_map), len(halo_src_map),
+                                typeid, flags))
+        return cls(h, Nq ** 3, Ne, NeA)
+
+    @classmethod
+    def structured(cls, NeX, NeY, NeZ, p, D1D, Lift, dom=(1.0, 1.0, 1.0),
+                   typeid=TYPE_HIP, flags=0):
+        h = C.c_void_p()
+        D1D, Lift = np.asfortranarray(D1D), np.asfortranarray(Lift)
+        _check(lib().dgk_create_structured(C.byref(h), NeX, NeY, NeZ, p, float(dom[0]),
+                                           float(dom[1]), float(dom[2]), _p(D1D),
+                                           _p(Lift), typeid, flags))
+        sz = mesh_sizes(NeX, NeY, NeZ, p)
+        return cls(h, sz["Np"], sz["Ne"], sz["NeA"])
+
+    def close(self):
+        if self._h:
+            lib().dgk_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _full(self, a):
+        """Accepts (Np,Ne) or (Np,NeA) and returns a (Np,NeA) F-array."""
+        a = np.asarray(a, dtype=np.float64)
+        if a.shape == (self.Np, self.NeA):
+            return np.asfortranarray(a)
+        if a.shape != (self.Np, self.Ne):
+            raise ValueError(f"field shape {a.shape} is neither (Np,Ne) nor (Np,NeA)")
+        f = np.zeros((self.Np, self.NeA), order="F")
+        f[:, :self.Ne] = a
+        return f
+
+    def set_fields(self, q, u, v, w):
+        q, u, v, w = (self._full(a) for a in (q, u, v, w))
+        _check(lib().dgk_set_fields(self._h, _p(q), _p(u), _p(v), _p(w)))
+
+    def get_field(self, which=FIELD_Q, owned=True):
+        out = np.zeros((self.Np, self.NeA), order="F")
+        _check(lib().dgk_get_field(self._h, which, _p(out)))
+        return out[:, :self.Ne] if owned else out
+
+    def field_ptr(self, which=FIELD_Q):
+        p = C.c_void_p()
+        _check(lib().dgk_field_ptr(self._h, which, C.byref(p)))
+        return p.value
+
+    def cal_tend(self):
+        _check(lib().dgk_cal_tend(self._h))
+        return self.get_field(FIELD_DQDT)
+
+    def rk_stage(self, stage, dt):
+        _check(lib().dgk_rk_stage(self._h, stage, float(dt)))
+
+    def step(self, dt):
+        _check(lib().dgk_step(self._h, float(dt)))
+
+    def run(self, dt, nstep, minmax=False):
+        mm = np.zeros((2, max(nstep, 1)), order="F") if minmax else None
+        _check(lib().dgk_run(self._h, float(dt), nstep, _p(mm)))
+        return mm[:, :nstep] if minmax else None
+
+    def synchronize(self):
+        _check(lib().dgk_synchronize(self._h))
+
+    def query(self, what):
+        v = C.c_int64(0)
+        _check(lib().dgk_query(self._h, what, C.byref(v)))
+        return v.value
+
+    @property
+    def compressed(self):
+        return bool(self.query(0))
+
+    def time_stages(self, dt, nstage):
+        ms = C.c_double(0.0)
+        _check(lib().dgk_time_stages(self._h, float(dt), nstage, C.byref(ms)))
+        return ms.value
+
+
+def advect3d_eq_cal_tend(q, u, v, w, D1D, D1D_tr, Lift, VMapM, VMapP, normal_fn,
+                         Escale, Fscale, Nq, Np, NfpTot, Ne, NeA, typeid=TYPE_OPT1):
+    """Stateless drop-in of advect3d_eq_cal_tend on host arrays (q,u,v,w with
+    halos, shape (Np,NeA)); returns dqdt(Np,Ne)."""
+    dqdt = np.zeros((Np, Ne), order="F")
+    arrs = [np.asfortranarray(a) for a in (q, u, v, w, D1D, D1D_tr, Lift)]
+    _check(lib().dgk_advect3d_eq_cal_tend(_p(dqdt), *[_p(a) for a in arrs], _p(VMapM),
+                                          _p(VMapP), _p(normal_fn), _p(Escale),
+                                          _p(Fscale), Nq, Np, NfpTot, Ne, NeA, typeid))
+    return dqdt
+
+
+def divlike_batched(Nq, nelem, typeid, Mat, Mat_tr, in_x, in_y, in_z, out_x, out_y,
+                    out_z, stream=None):
+    """Device pointers (ints) for every array argument."""
+    _check(lib().dgk_divlike_dirXYZ_batched(Nq, nelem, typeid, Mat, Mat_tr, in_x, in_y,
+                                            in_z, out_x, out_y, out_z, stream))
+
+
+def lift_batched(Nq, nelem, Lift, fin, fout, stream=None):
+    _check(lib().dgk_lift_hexahedral_batched(Nq, nelem, Lift, fin, fout, stream))

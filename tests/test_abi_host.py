@@ -1,0 +1,108 @@
+"""libdgk.so on the CPU: it loads, exports every entry point include/dgk.h
+declares, and its host-side producers (operator reader, mesh builder,
+selector) agree bit for bit with the oracle and the reference's golden
+digests.  No kernel is launched here."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import cases as CS
+import dgk
+import oracle as O
+
+HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include", "dgk.h")
+
+
+def declared():
+    txt = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?\w+\s*\*?\s*(dgk_\w+)\s*\(", txt, re.M)))
+
+
+def test_exports_every_declared_symbol():
+    L = dgk.lib()
+    names = declared()
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    assert set(names) == set(dgk.EXPORTS)
+
+
+def test_selector_names():
+    """dg_optr_kernel_setup (mod_dg_optr_kernel.f90:45-58) plus "HIP"."""
+    assert dgk.optr_kernel_setup("GENERAL") == 1
+    assert dgk.optr_kernel_setup("OPT1") == 2
+    assert dgk.optr_kernel_setup("HIP") == 3
+    assert dgk.optr_kernel_setup("HIP     ") == 3  # character(len=8) padding
+    with pytest.raises(dgk.DGKError) as e:
+        dgk.optr_kernel_setup("CUDA")
+    assert e.value.code == dgk.ERR_OPTTYPE
+
+
+@pytest.mark.parametrize("p", range(1, 16))
+def test_read_operator_matches_oracle(p):
+    a = dgk.read_operator(p)
+    b = O.read_operator(p)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+
+
+def test_read_operator_errors(tmp_path):
+    with pytest.raises(dgk.DGKError) as e:
+        dgk.read_operator(16)
+    assert e.value.code == dgk.ERR_ORDER
+    with pytest.raises(dgk.DGKError) as e:
+        dgk.read_operator(3, str(tmp_path))
+    assert e.value.code == dgk.ERR_IO
+    (tmp_path / "p2.dat").write_text("-1 0 1\n")  # truncated file
+    with pytest.raises(dgk.DGKError) as e:
+        dgk.read_operator(2, str(tmp_path))
+    assert e.value.code == dgk.ERR_IO
+
+
+@pytest.mark.parametrize("mesh", CS.MESH_CASES, ids=lambda m: CS.case_key(*m))
+def test_mesh_setup_matches_reference(golden, mesh):
+    t = dgk.mesh_setup(*mesh)
+    g = golden["mesh"][CS.case_key(*mesh)]
+    for k in ("VMapM", "VMapP", "halo_src_map", "normal_fn", "Escale", "Fscale", "pos_en"):
+        assert CS.digest(t[k]) == g[k], k
+
+
+@pytest.mark.parametrize("mesh", [(1, 1, 1, 1), (5, 1, 3, 2), (4, 4, 4, 7), (2, 5, 3, 11)])
+def test_mesh_setup_matches_oracle(mesh):
+    t = dgk.mesh_setup(*mesh)
+    m = O.Mesh(*mesh)
+    for k in ("VMapM", "VMapP", "halo_src_map", "normal_fn", "Escale", "Fscale", "pos_en"):
+        assert np.array_equal(t[k], getattr(m, k)), k
+
+
+def test_mesh_sizes_and_errors():
+    s = dgk.mesh_sizes(64, 64, 64, 7)
+    assert s == dict(Nq=8, Np=512, Nfp=64, NfpTot=384, Ne=262144, NeA=286720,
+                     NhaloNode=1572864)
+    with pytest.raises(dgk.DGKError):
+        dgk.mesh_sizes(0, 4, 4, 3)
+    with pytest.raises(dgk.DGKError) as e:
+        dgk.mesh_sizes(4, 4, 4, 0)
+    assert e.value.code == dgk.ERR_ORDER
+    with pytest.raises(dgk.DGKError):  # Np*NeA beyond the int32 index range
+        dgk.mesh_sizes(160, 160, 160, 7)
+
+
+def test_create_rejects_bad_tables_before_device_use():
+    """Index validation happens on the host, before any allocation."""
+    m = dgk.mesh_setup(2, 2, 2, 2)
+    x1D, D, L = dgk.read_operator(2)
+    bad = m["VMapP"].copy(order="F")
+    bad[0, 0] = m["Np"] * m["NeA"] + 5
+    with pytest.raises(dgk.DGKError) as e:
+        dgk.Solver.from_tables(3, m["Ne"], m["NeA"], D, L, m["VMapM"], bad, m["normal_fn"],
+                               m["Escale"], m["Fscale"], m["halo_src_map"])
+    assert e.value.code == dgk.ERR_ARG
+    with pytest.raises(dgk.DGKError) as e:
+        dgk.Solver.from_tables(3, m["Ne"], m["NeA"], D, L, m["VMapM"], m["VMapP"],
+                               m["normal_fn"], m["Escale"], m["Fscale"], m["halo_src_map"],
+                               typeid=7)
+    assert e.value.code == dgk.ERR_OPTTYPE

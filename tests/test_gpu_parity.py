@@ -1,0 +1,235 @@
+"""Device path (libdgk.so on the MI355X) against the reference.
+
+Exact variants (default, DGOptrKernel_OptType "HIP") must reproduce the
+reference's FP64 results bit for bit: digests equal to the golden fixtures
+written from the compiled reference, and arrays equal to the oracle's on the
+same inputs.  FMA variants must stay within 1e-12 relative L2 after one step
+and 1e-10 after nstep.
+"""
+import numpy as np
+import pytest
+
+import cases as CS
+import dgk
+import oracle as O
+from conftest import full_field, say
+from dgk import ic
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def oracle_fields(m, q, u, v, w):
+    F = []
+    for a in (q, u, v, w):
+        f = m.field()
+        f[:, :m.Ne] = a
+        m.update_halo(f)
+        F.append(f)
+    return F
+
+
+def to_dev(torch, a):
+    return torch.from_numpy(np.ascontiguousarray(np.asarray(a).ravel(order="F"))).cuda()
+
+
+def from_dev(t, shape):
+    return t.cpu().numpy().reshape(shape, order="F")
+
+
+# ---------------------------------------------------------------------------
+# batched element operators
+
+
+@pytest.mark.parametrize("p", range(1, 16))
+@pytest.mark.parametrize("tname", ["GENERAL", "OPT1"])
+def test_batched_operators_bit_exact(gpu, golden, p, tname):
+    torch = gpu
+    say(f"[operators] p={p} {tname}")
+    Nq = p + 1
+    Np = Nq ** 3
+    _, D, L = dgk.read_operator(p)
+    ix, iy, iz, fb = CS.optr_inputs(p)
+    n = CS.OPTR_NELEM
+    dD, dDt, dL = to_dev(torch, D), to_dev(torch, np.asfortranarray(D.T)), to_dev(torch, L)
+    dix, diy, diz, dfb = (to_dev(torch, a) for a in (ix, iy, iz, fb))
+    ox, oy, oz, ol = (torch.zeros(Np * n, dtype=torch.float64, device="cuda") for _ in range(4))
+    tid = dgk.optr_kernel_setup(tname)
+    dgk.divlike_batched(Nq, n, tid, dD.data_ptr(), dDt.data_ptr(), dix.data_ptr(),
+                        diy.data_ptr(), diz.data_ptr(), ox.data_ptr(), oy.data_ptr(),
+                        oz.data_ptr())
+    dgk.lift_batched(Nq, n, dL.data_ptr(), dfb.data_ptr(), ol.data_ptr())
+    torch.cuda.synchronize()
+    g = golden["optr"][CS.case_key(p, tname)]
+    for t, k in ((ox, "x"), (oy, "y"), (oz, "z"), (ol, "lift")):
+        assert CS.digest(from_dev(t, (Np, n))) == g[k]["sha256"], k
+
+
+# ---------------------------------------------------------------------------
+# tendency (advect3d_eq_cal_tend)
+
+
+def _solver_variants(ne, p, D, L, typeid, flags=0):
+    m = dgk.mesh_setup(*ne, p)
+    yield "structured", dgk.Solver.structured(*ne, p, D, L, typeid=typeid, flags=flags)
+    args = (p + 1, m["Ne"], m["NeA"], D, L, m["VMapM"], m["VMapP"], m["normal_fn"],
+            m["Escale"], m["Fscale"], m["halo_src_map"])
+    s = dgk.Solver.from_tables(*args, typeid=typeid, flags=flags)
+    assert s.compressed
+    yield "tables-compressed", s
+    s = dgk.Solver.from_tables(*args, typeid=typeid, flags=flags | dgk.FLAG_FORCE_TABLES)
+    assert not s.compressed
+    yield "tables", s
+
+
+@pytest.mark.parametrize("case", CS.TEND_CASES, ids=lambda c: CS.case_key(*c[0], c[1]))
+@pytest.mark.parametrize("tname", ["GENERAL", "OPT1"])
+def test_tendency_bit_exact(gpu, golden, case, tname):
+    ne, p = case
+    _, D, L = dgk.read_operator(p)
+    q, u, v, w = CS.tend_inputs(ne, p)
+    want = golden["tend"][CS.case_key(*ne, p, tname)]["sha256"]
+    tid = dgk.optr_kernel_setup(tname)
+    for name, s in _solver_variants(ne, p, D, L, tid):
+        say(f"[tend] {ne} p={p} {tname} {name}")
+        s.set_fields(q, u, v, w)
+        assert CS.digest(s.cal_tend()) == want, name
+        s.close()
+    say(f"[tend] {ne} p={p} {tname} stateless drop-in")
+    m = dgk.mesh_setup(*ne, p)
+    F = [full_field(a, m["NeA"]) for a in (q, u, v, w)]
+    # the stateless call expects halos filled, as the reference's caller does
+    om = O.Mesh(*ne, p)
+    for f in F:
+        om.update_halo(f)
+    d = dgk.advect3d_eq_cal_tend(*F, D, np.asfortranarray(D.T), L, m["VMapM"], m["VMapP"],
+                                 m["normal_fn"], m["Escale"], m["Fscale"], p + 1, m["Np"],
+                                 m["NfpTot"], m["Ne"], m["NeA"], typeid=tid)
+    assert CS.digest(d) == want
+
+
+# ---------------------------------------------------------------------------
+# trajectories (main.f90 step loop)
+
+
+@pytest.mark.parametrize("case", CS.RUN_CASES, ids=lambda c: CS.case_key(*c[0], c[1], c[2]))
+@pytest.mark.parametrize("tname", ["GENERAL", "OPT1"])
+def test_trajectory_bit_exact(gpu, golden, case, tname):
+    ne, p, nstep, vel, kind = case
+    x1D, D, L = dgk.read_operator(p)
+    m = dgk.mesh_setup(*ne, p, x1D=x1D, tables=False)
+    q, u, v, w, dt = CS.run_inputs(ne, p, vel, kind, m["pos_en"])
+    g = golden["run"][CS.case_key(*ne, p, nstep, tname)]
+    tid = dgk.optr_kernel_setup(tname)
+    for name, s in _solver_variants(ne, p, D, L, tid):
+        say(f"[run] {ne} p={p} nstep={nstep} {tname} {name}")
+        s.set_fields(q, u, v, w)
+        mm = s.run(dt, nstep, minmax=True)
+        assert CS.digest(s.get_field()) == g["q"]["sha256"], name
+        assert [[float(a).hex() for a in col] for col in mm.T] == g["minmax"], name
+        s.close()
+
+
+@pytest.mark.parametrize("case", CS.RUN_CASES[:3], ids=lambda c: CS.case_key(*c[0], c[1], c[2]))
+def test_trajectory_fma_tolerance(gpu, case):
+    """FMA contractions: rel-L2 <= 1e-12 after one step, <= 1e-10 after nstep
+    against the oracle (OPT1 order)."""
+    ne, p, nstep, vel, kind = case
+    _, D, L = dgk.read_operator(p)
+    om = O.Mesh(*ne, p)
+    q, u, v, w, dt = CS.run_inputs(ne, p, vel, kind, om.pos_en)
+    for steps, tol in ((1, 1e-12), (nstep, 1e-10)):
+        say(f"[fma] {ne} p={p} steps={steps}")
+        F = oracle_fields(om, q, u, v, w)
+        om.run(*F, dt, steps, O.OPT1)
+        s = dgk.Solver.structured(*ne, p, D, L, flags=dgk.FLAG_FMA)
+        s.set_fields(q, u, v, w)
+        s.run(dt, steps)
+        err = rel_l2(s.get_field(), F[0][:, :om.Ne])
+        assert err <= tol, err
+        assert err > 0.0  # the fused variant really is a different rounding
+        s.close()
+
+
+def test_permuted_element_numbering_uses_tables(gpu):
+    """A valid mesh whose elements are renumbered is not the structured
+    numbering, so the solver keeps the per-node tables; results equal the
+    oracle run on the same permuted tables bit for bit."""
+    ne, p, nstep = (3, 2, 2), 4, 4
+    om = O.Mesh(*ne, p)
+    Np, Ne, NeA = om.Np, om.Ne, om.NeA
+    perm = np.random.default_rng(5).permutation(Ne)  # old element e -> new perm[e]
+
+    def remap(idx1):  # 1-based flat indices into (Np,NeA)
+        i = idx1 - 1
+        own = i < Np * Ne
+        e, n = i // Np, i % Np
+        out = i.copy()
+        out[own] = n[own] + perm[e[own]] * Np
+        return (out + 1).astype(np.int32)
+
+    inv = np.argsort(perm)
+    VM = np.asfortranarray(remap(om.VMapM)[:, inv])
+    VP = np.asfortranarray(remap(om.VMapP)[:, inv])
+    nrm = np.asfortranarray(om.normal_fn[:, inv, :])
+    Es = np.asfortranarray(om.Escale[:, inv, :])
+    Fs = np.asfortranarray(om.Fscale[:, inv])
+    halo = remap(om.halo_src_map)
+    q0 = CS.uniform(77, (Np, Ne))
+    vel = [CS.uniform(78 + d, (Np, Ne)) for d in range(3)]
+    qn, un, vn, wn = (np.asfortranarray(a[:, inv]) for a in (q0, *vel))
+    # oracle on the permuted tables
+    om.VMapM, om.VMapP, om.normal_fn, om.Escale, om.Fscale, om.halo_src_map = VM, VP, nrm, Es, Fs, halo
+    F = oracle_fields(om, qn, un, vn, wn)
+    dt = 2e-3
+    om.run(*F, dt, nstep, O.OPT1)
+    say("[permuted] tables path")
+    s = dgk.Solver.from_tables(p + 1, Ne, NeA, om.D1D, om.Lift, VM, VP, nrm, Es, Fs, halo)
+    assert not s.compressed
+    s.set_fields(qn, un, vn, wn)
+    s.run(dt, nstep)
+    assert np.array_equal(s.get_field(), F[0][:, :Ne])
+    s.close()
+
+
+def test_graph_step_equals_stagewise(gpu):
+    """Replays of the captured time step, with eager work and host syncs in
+    between, equal the stage-by-stage launches bit for bit."""
+    ne, p = (4, 3, 5), 5
+    x1D, D, L = dgk.read_operator(p)
+    m = dgk.mesh_setup(*ne, p, x1D=x1D, tables=False)
+    q = ic.gaussian_noise(m["pos_en"], 31)
+    vel = [np.full_like(q, c) for c in (0.4, 0.9, -0.2)]
+    dt = ic.cfl_dt(p, (0.4, 0.9, -0.2), ne)
+    a = dgk.Solver.structured(*ne, p, D, L)
+    b = dgk.Solver.structured(*ne, p, D, L)
+    a.set_fields(q, *vel)
+    b.set_fields(q, *vel)
+    for it in range(5):
+        say(f"[graph] replay {it}")
+        a.step(dt)
+        a.synchronize()
+        for st in (1, 2, 3):
+            b.rk_stage(st, dt)
+        b.synchronize()
+        _ = b.cal_tend()  # eager work between replays
+        assert np.array_equal(a.get_field(), b.get_field()), it
+    a.close()
+    b.close()
+
+
+def test_state_errors(gpu):
+    _, D, L = dgk.read_operator(2)
+    s = dgk.Solver.structured(2, 2, 2, 2, D, L)
+    with pytest.raises(dgk.DGKError) as e:
+        s.step(1e-3)
+    assert e.value.code == dgk.ERR_STATE
+    s.set_fields(*(np.ones((27, 8), order="F") for _ in range(4)))
+    with pytest.raises(dgk.DGKError):
+        s.rk_stage(4, 1e-3)
+    with pytest.raises(ValueError):
+        s.set_fields(*(np.ones((27, 7), order="F") for _ in range(4)))
+    s.close()
