@@ -35,6 +35,14 @@ METRIC = "GDOF-updates/s per RK stage at p=7, 64^3 elems; rel-L2 err vs FP64 ora
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_DOF = 48  # per DOF-update and stage, per-node FP64 velocity
 PROFILES = os.path.join(REPO, "profiles")
+# solver variants: flags of dgk_create_structured
+VARIANTS = {
+    "exact": 0,                                        # bit-identical to the reference
+    "fma": dgk.FLAG_FMA,                               # fused multiply-adds
+    "fma_factored": dgk.FLAG_FMA | dgk.FLAG_FACTORED_LIFT,
+    "exact_node": dgk.FLAG_NODE_KERNEL,                # thread-per-node kernel
+    "fma_node": dgk.FLAG_FMA | dgk.FLAG_NODE_KERNEL,
+}
 
 
 def log(msg):
@@ -48,7 +56,8 @@ def lgl_weights(x1D):
 
 
 def mass(q, w3, J):
-    return float(J * np.dot(w3, q.sum(axis=1)))
+    """LGL-quadrature integral of q and of |q| over the domain."""
+    return float(J * np.dot(w3, q.sum(axis=1))), float(J * np.dot(w3, np.abs(q).sum(axis=1)))
 
 
 def cpu_baseline(threads, nstep):
@@ -102,7 +111,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="c3_64cube_p7", choices=sorted(ic.WORKLOADS))
-    ap.add_argument("--variant", default="exact", choices=("exact", "fma"))
+    ap.add_argument("--variant", default="exact", choices=sorted(VARIANTS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-steps", type=int, default=5)
@@ -123,12 +132,12 @@ def main():
     u, v, w = (np.full_like(q, c) for c in vel)
     dt = ic.cfl_dt(p, vel, ne)
     log(f"initial field generated in {time.perf_counter() - t0:.1f} s")
-    flags = dgk.FLAG_FMA if args.variant == "fma" else 0
-    s = dgk.Solver.structured(*ne, p, D, L, flags=flags)
+    s = dgk.Solver.structured(*ne, p, D, L, flags=VARIANTS[args.variant])
+    kernel_name = "pencil_kernel" if s.kernel_kind == "pencil" else "stage_kernel"
     s.set_fields(q, u, v, w)
     w3 = np.einsum("i,j,k->ijk", *(lgl_weights(x1D),) * 3).ravel(order="F")
     J = (1.0 / ne[0]) * (1.0 / ne[1]) * (1.0 / ne[2]) / 8.0
-    mass0 = mass(q, w3, J)
+    mass0, mass_abs = mass(q, w3, J)
     del q, u, v, w
     Ne, Np = s.Ne, s.Np
 
@@ -153,7 +162,9 @@ def main():
     achieved = bytes_launch / (ms_stage * 1e-3) / 1e9
     qf = s.get_field()
     finite = bool(np.isfinite(qf).all())
-    drift = abs(mass(qf, w3, J) - mass0) / abs(mass0) if mass0 else None
+    # the sine-mode field has (near) zero mean: drift is measured against the
+    # integral of |q|
+    drift = abs(mass(qf, w3, J)[0] - mass0) / mass_abs
     s.close()
     log(f"stage kernel {ms_stage * 1e3:.1f} us/launch -> {achieved:.0f} GB/s algorithmic")
 
@@ -183,7 +194,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": traffic_from_profiles(args.workload, args.variant),
-                     "kernel": "stage_kernel", "kernel_us": round(ms_stage * 1e3, 2),
+                     "kernel": kernel_name, "kernel_us": round(ms_stage * 1e3, 2),
                      "algorithmic_bytes_per_launch": bytes_launch},
         "cpu_baseline": cpu,
         "checks": {"finite": finite, "mass_drift_rel": drift},

@@ -47,6 +47,11 @@ extern "C" {
                                       bit-identical to the reference)       */
 #define DGK_FLAG_FORCE_TABLES 0x2  /* always use the per-node VMapM/VMapP,
                                       normal, Fscale and Escale tables       */
+#define DGK_FLAG_NODE_KERNEL 0x4   /* thread-per-node stage kernel even where
+                                      the pencil kernel applies (Nq 2, 4, 8) */
+#define DGK_FLAG_FACTORED_LIFT 0x8 /* lift with the 1D vectors
+                                      Lift(:,1,1,4) and Lift(:,1,1,2) (exact
+                                      to ~1e-14 relative, not bit-identical) */
 
 /* Fields addressable through dgk_get_field / dgk_field_ptr. */
 #define DGK_FIELD_Q 0
@@ -136,7 +141,8 @@ int dgk_step(dgk_solver *s, double dt);
 int dgk_run(dgk_solver *s, double dt, int nstep, double *minmax);
 int dgk_synchronize(dgk_solver *s);
 /* what: 0 = connectivity mode (1 compressed, 0 tables), 1 = Ne, 2 = Np,
- * 3 = NeA, 4 = grid size of the stage kernel, 5 = block size. */
+ * 3 = NeA, 4 = grid size of the stage kernel, 5 = block size, 6 = stage
+ * kernel kind (0 thread-per-node, 1 pencil), 7 = LDS bytes per workgroup. */
 int dgk_query(dgk_solver *s, int what, int64_t *value);
 /* Times `nstage` consecutive stage launches with HIP events on the solver's
  * stream; returns the mean milliseconds per stage launch. */

@@ -3,6 +3,7 @@
 // summation order, halo gathers and the hipGraph-captured time step.
 #include "dgk_internal.hpp"
 #include "dgk_kernels.hpp"
+#include "dgk_pencil.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -23,19 +24,82 @@ namespace {
 
 using StageFn = void (*)(const StageParams);
 
+// Device copies of the 1D operator in row-major order (D(k,l) at
+// [k*Nq + l], read a row at a time by the pencil kernel) and of the vectors
+// [l_-, l_+] of the factored lift.
+struct HostOps {
+  const double *Drow;
+  const double *lfac;
+};
+
+struct StageKernel;
+using LaunchFn = void (*)(const StageKernel &, const StageParams &, const HostOps &,
+                          const double *Lift, hipStream_t);
+
 struct StageKernel {
-  StageFn fn = nullptr;
+  const void *kernel = nullptr;  // for attributes and occupancy
+  LaunchFn launch = nullptr;
   int nt = 0;
   size_t lds = 0;
+  int epb = 1;       // elements per workgroup (node kernel) or wavefront (pencil)
+  bool pencil = false;
+  long long grid = 0;
 };
+
+template <int NQ, int ORDER, bool FMA, bool STRUCT>
+void launch_node(const StageKernel &k, const StageParams &P, const HostOps &, const double *,
+                 hipStream_t st) {
+  hipLaunchKernelGGL((stage_kernel<NQ, ORDER, FMA, STRUCT>), dim3((unsigned)k.grid),
+                     dim3(k.nt), k.lds, st, P);
+}
 
 template <int NQ, int ORDER, bool FMA, bool STRUCT>
 StageKernel make_stage() {
   StageKernel k;
-  k.fn = stage_kernel<NQ, ORDER, FMA, STRUCT>;
+  k.kernel = (const void *)stage_kernel<NQ, ORDER, FMA, STRUCT>;
+  k.launch = launch_node<NQ, ORDER, FMA, STRUCT>;
   k.nt = StageShape<NQ>::NT;
   k.lds = StageShape<NQ>::LDS;
+  k.epb = StageShape<NQ>::EPB;
   return k;
+}
+
+template <int NQ, int ORDER, bool FMA, bool FACT>
+void launch_pencil(const StageKernel &k, const StageParams &P, const HostOps &H,
+                   const double *Lift, hipStream_t st) {
+  hipLaunchKernelGGL((pencil_kernel<NQ, ORDER, FMA, FACT>), dim3((unsigned)k.grid), dim3(64),
+                     k.lds, st, P, H.Drow, Lift, H.lfac);
+}
+
+template <int NQ, int ORDER, bool FMA, bool FACT>
+StageKernel make_pencil() {
+  StageKernel k;
+  k.kernel = (const void *)pencil_kernel<NQ, ORDER, FMA, FACT>;
+  k.launch = launch_pencil<NQ, ORDER, FMA, FACT>;
+  k.nt = 64;
+  k.lds = PencilShape<NQ>::LDS;
+  k.epb = PencilShape<NQ>::EPW;
+  k.pencil = true;
+  return k;
+}
+
+template <int NQ>
+StageKernel pick_pencil_nq(int order, bool fma, bool fact) {
+  if (order == kGeneral) {
+    if (fma) return fact ? make_pencil<NQ, kGeneral, true, true>() : make_pencil<NQ, kGeneral, true, false>();
+    return fact ? make_pencil<NQ, kGeneral, false, true>() : make_pencil<NQ, kGeneral, false, false>();
+  }
+  if (fma) return fact ? make_pencil<NQ, kOpt1, true, true>() : make_pencil<NQ, kOpt1, true, false>();
+  return fact ? make_pencil<NQ, kOpt1, false, true>() : make_pencil<NQ, kOpt1, false, false>();
+}
+
+StageKernel pick_pencil(int Nq, int order, bool fma, bool fact) {
+  switch (Nq) {
+  case 2: return pick_pencil_nq<2>(order, fma, fact);
+  case 4: return pick_pencil_nq<4>(order, fma, fact);
+  case 8: return pick_pencil_nq<8>(order, fma, fact);
+  default: return StageKernel{};
+  }
 }
 
 template <int NQ>
@@ -101,9 +165,11 @@ struct dgk_solver {
   int order = kOpt1;
   bool fma = false;
   bool structured = false;
+  int flags = 0;
+  HostOps ops;
   double *q[2] = {nullptr, nullptr};
   double *q0 = nullptr, *u = nullptr, *v = nullptr, *w = nullptr, *dqdt = nullptr;
-  double *D = nullptr, *Lift = nullptr;
+  double *D = nullptr, *Lift = nullptr, *lfac = nullptr, *Drow = nullptr;
   int32_t *nbr = nullptr, *vmapM = nullptr, *vmapP = nullptr, *halo = nullptr;
   double *nrm = nullptr, *fscale = nullptr, *escale = nullptr;
   double fs[6] = {0, 0, 0, 0, 0, 0}, es[3] = {0, 0, 0};
@@ -112,7 +178,6 @@ struct dgk_solver {
   int cur = 0;
   hipStream_t stream = nullptr;
   StageKernel sk;
-  int grid = 0;
   hipGraphExec_t graph[2] = {nullptr, nullptr};
   double graph_dt = 0.0;
   bool fields_set = false;
@@ -121,24 +186,34 @@ struct dgk_solver {
 namespace {
 
 int configure_launch(dgk_solver *s) {
-  s->sk = pick_stage(s->Nq, s->order, s->fma, s->structured);
-  if (!s->sk.fn) return fail(DGK_ERR_ORDER, "Nq %d not supported", s->Nq);
-  HIPOK(hipFuncSetAttribute((const void *)s->sk.fn,
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)s->sk.lds));
+  const bool use_pencil = s->structured && !(s->flags & DGK_FLAG_NODE_KERNEL) &&
+                          (s->Nq == 2 || s->Nq == 4 || s->Nq == 8);
+  if (use_pencil)
+    s->sk = pick_pencil(s->Nq, s->order, s->fma, (s->flags & DGK_FLAG_FACTORED_LIFT) != 0);
+  else
+    s->sk = pick_stage(s->Nq, s->order, s->fma, s->structured);
+  if (!s->sk.kernel) return fail(DGK_ERR_ORDER, "Nq %d not supported", s->Nq);
+  if ((s->flags & DGK_FLAG_FACTORED_LIFT) && !use_pencil)
+    return fail(DGK_ERR_ARG, "DGK_FLAG_FACTORED_LIFT needs the pencil kernel (Nq 2, 4 or 8, "
+                             "compressed connectivity)");
+  HIPOK(hipFuncSetAttribute(s->sk.kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)s->sk.lds));
+  const long long ngroups = (s->Ne + s->sk.epb - 1) / s->sk.epb;
+  if (s->sk.pencil) {
+    // one workgroup per element group, remapped per XCD inside the kernel
+    s->sk.grid = (ngroups + kNumXcd - 1) / kNumXcd * kNumXcd;
+    return DGK_OK;
+  }
   int dev = 0, ncu = 0, per_cu = 0;
   HIPOK(hipGetDevice(&dev));
   HIPOK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
-  HIPOK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)s->sk.fn,
-                                                     s->sk.nt, s->sk.lds));
+  HIPOK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, s->sk.kernel, s->sk.nt,
+                                                     s->sk.lds));
   if (per_cu < 1) per_cu = 1;
-  const long long np = s->Np;
-  const long long e_per_group = np >= 256 ? 1 : 256 / np;
-  const long long ngroups = (s->Ne + e_per_group - 1) / e_per_group;
   long long grid = (long long)per_cu * ncu;
   const long long need = (ngroups + kNumXcd - 1) / kNumXcd * kNumXcd;
   if (grid > need) grid = need;
-  grid = std::max<long long>(kNumXcd, grid / kNumXcd * kNumXcd);
-  s->grid = (int)grid;
+  s->sk.grid = std::max<long long>(kNumXcd, grid / kNumXcd * kNumXcd);
   return DGK_OK;
 }
 
@@ -198,7 +273,7 @@ int launch_stage(dgk_solver *s, int mode, double a, double b, double dt) {
     if (rc) return rc;
   }
   StageParams P = make_params(s, mode, a, b, dt);
-  hipLaunchKernelGGL(s->sk.fn, dim3(s->grid), dim3(s->sk.nt), s->sk.lds, s->stream, P);
+  s->sk.launch(s->sk, P, s->ops, s->Lift, s->stream);
   HIPOK(hipGetLastError());
   if (mode != kModeTend) s->cur ^= 1;
   return DGK_OK;
@@ -263,10 +338,25 @@ bool compress_tables(int Nq, long long Ne, const int32_t *VMapM, const int32_t *
 
 int finish_create(dgk_solver *s, const double *D1D, const double *Lift) {
   int rc;
-  const size_t nq2 = (size_t)s->Nq * s->Nq, nlift = (size_t)6 * s->Np;
-  if ((rc = dalloc(&s->D, nq2)) || (rc = dalloc(&s->Lift, nlift))) return rc;
+  const int Nq = s->Nq, Np = s->Np;
+  const size_t nq2 = (size_t)Nq * Nq, nlift = (size_t)6 * Np;
+  if ((rc = dalloc(&s->D, nq2)) || (rc = dalloc(&s->Lift, nlift)) ||
+      (rc = dalloc(&s->lfac, 2 * (size_t)Nq)) || (rc = dalloc(&s->Drow, nq2)))
+    return rc;
   HIPOK(hipMemcpy(s->D, D1D, nq2 * sizeof(double), hipMemcpyHostToDevice));
   HIPOK(hipMemcpy(s->Lift, Lift, nlift * sizeof(double), hipMemcpyHostToDevice));
+  std::vector<double> dr(nq2);
+  for (int k = 0; k < Nq; ++k)
+    for (int l = 0; l < Nq; ++l) dr[(size_t)k * Nq + l] = D1D[k + (size_t)l * Nq];
+  HIPOK(hipMemcpy(s->Drow, dr.data(), nq2 * sizeof(double), hipMemcpyHostToDevice));
+  s->ops.Drow = s->Drow;
+  std::vector<double> lf(2 * (size_t)Nq);
+  for (int i = 0; i < Nq; ++i) {
+    lf[i] = Lift[i + (size_t)Np * 3];       // face 4 (-x): l_-(i) = Lift(i,1,1,4)
+    lf[Nq + i] = Lift[i + (size_t)Np * 1];  // face 2 (+x): l_+(i) = Lift(i,1,1,2)
+  }
+  HIPOK(hipMemcpy(s->lfac, lf.data(), lf.size() * sizeof(double), hipMemcpyHostToDevice));
+  s->ops.lfac = s->lfac;
   if ((rc = alloc_fields(s))) return rc;
   if ((rc = configure_launch(s))) return rc;
   HIPOK(hipStreamSynchronize(s->stream));
@@ -289,7 +379,7 @@ extern "C" {
 int dgk_destroy(dgk_solver *s) {
   if (!s) return DGK_OK;
   drop_graphs(s);
-  void *bufs[] = {s->q[0], s->q[1], s->q0, s->u, s->v, s->w, s->dqdt, s->D, s->Lift,
+  void *bufs[] = {s->q[0], s->q[1], s->q0, s->u, s->v, s->w, s->dqdt, s->D, s->Lift, s->lfac, s->Drow,
                   s->nbr, s->vmapM, s->vmapP, s->halo, s->nrm, s->fscale, s->escale,
                   s->part, s->mm};
   for (void *b : bufs)
@@ -336,6 +426,7 @@ int dgk_create(dgk_solver **out, int Nq, int Ne, int NeA, int NfpTot, const doub
   s->NhaloNode = NhaloNode;
   s->order = order;
   s->fma = (flags & DGK_FLAG_FMA) != 0;
+  s->flags = flags;
   int rc = DGK_OK;
   if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
     delete s;
@@ -388,6 +479,7 @@ int dgk_create_structured(dgk_solver **out, int NeX, int NeY, int NeZ, int p, do
   s->NhaloNode = sz[6];
   s->order = order;
   s->fma = (flags & DGK_FLAG_FMA) != 0;
+  s->flags = flags;
   s->structured = true;
   if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
     delete s;
@@ -547,8 +639,10 @@ int dgk_query(dgk_solver *s, int what, int64_t *value) {
   case 1: *value = s->Ne; break;
   case 2: *value = s->Np; break;
   case 3: *value = s->NeA; break;
-  case 4: *value = s->grid; break;
+  case 4: *value = s->sk.grid; break;
   case 5: *value = s->sk.nt; break;
+  case 6: *value = s->sk.pencil ? 1 : 0; break;
+  case 7: *value = (int64_t)s->sk.lds; break;
   default: return fail(DGK_ERR_ARG, "unknown query %d", what);
   }
   return DGK_OK;

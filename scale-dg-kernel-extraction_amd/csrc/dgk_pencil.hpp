@@ -1,0 +1,263 @@
+// Pencil form of the fused DG advection stage for Nq in {2, 4, 8}: one
+// 64-lane wavefront per workgroup, Nq^2 lanes per element, each lane owning
+// one line of Nq nodes of its element at a time.
+//
+// Lane (i,j) loads the z-line (i,j,0..Nq-1) of q,u,v,w,q0: for every k the
+// wavefront reads Nq^2 consecutive doubles, so all streaming loads and stores
+// are fully coalesced.
+//   z phase  registers: f_z = q w along the z-line, the z sweep and both
+//            z-face fluxes need no data from other lanes.
+//   y phase  q and v staged in LDS are read back along y-lines (i,k): the y
+//            sweep and the two y-face fluxes; D_y returns to LDS.
+//   x phase  q and u along x-lines (j,k): the x sweep and the two x-face
+//            fluxes; D_x returns to LDS.
+//   final    z-lines again: D_x, D_y and the x/y face fluxes come from LDS,
+//            the lift, the combine and the SSP-RK3 update finish in
+//            registers, and the new state is stored coalesced.
+// Every LDS position belongs to exactly one y-line and one x-line, so a line
+// may overwrite the slots it has read without a barrier.  LDS buffers pad
+// every x-line by one double (index n + n/Nq): z-line writes, y-line and
+// x-line reads of a wavefront then fall in distinct banks (but for one pair).
+// The 1D operator D is read one row at a time into SGPRs with scalar loads,
+// so every sweep multiplies by SGPR operands.  All neighbour-face
+// loads are issued with the element's own loads at the start.  Arithmetic
+// order is that of stage_kernel (the reference's order in exact variants).
+#pragma once
+
+#include "dgk_kernels.hpp"
+
+namespace dgk {
+
+template <int NQ>
+struct PencilShape {
+  static_assert(NQ == 2 || NQ == 4 || NQ == 8, "pencil kernel: Nq^2 must divide 64");
+  static constexpr int NP = NQ * NQ * NQ;
+  static constexpr int NFP = NQ * NQ;
+  static constexpr int EPW = 64 / NFP;         // elements per wavefront
+  static constexpr int PAD = NP + NP / NQ;     // padded volume buffer
+  static constexpr int FPAD = NFP + NFP / NQ;  // padded face buffer
+  static constexpr int PER_ELEM = 3 * PAD + 4 * FPAD;
+  static constexpr size_t LDS = sizeof(double) * EPW * PER_ELEM;
+};
+
+template <int NQ>
+__device__ __forceinline__ int pidx(int n) { return n + n / NQ; }
+
+// Row k of the 1D operator, D(k,0..NQ-1), from the row-major copy Drow into
+// SGPRs with one scalar load.  The load is written out (with its wait in the
+// same statement) because left to itself the compiler loads all NQ^2
+// coefficients at the top of the kernel and spills them to VGPR lanes, which
+// costs more VALU issue slots than the sweeps themselves.
+// `after` is a value the row must not be loaded before (the previous
+// output): it keeps one row of SGPRs live at a time.
+template <int NQ>
+__device__ __forceinline__ void sgpr_row(const double *Drow, int k, double after,
+                                         double (&r)[NQ]) {
+  const double *p = Drow + k * NQ;
+  if constexpr (NQ == 8) {
+    typedef int v16 __attribute__((ext_vector_type(16)));
+    v16 x;
+    asm volatile("s_load_dwordx16 %0, %1, 0x0\n\ts_waitcnt lgkmcnt(0)"
+                 : "=s"(x)
+                 : "s"(p), "v"(after));
+#pragma unroll
+    for (int l = 0; l < 8; ++l)
+      r[l] = __builtin_bit_cast(double, (unsigned long long)(unsigned)x[2 * l] |
+                                            ((unsigned long long)(unsigned)x[2 * l + 1] << 32));
+  } else if constexpr (NQ == 4) {
+    typedef int v8 __attribute__((ext_vector_type(8)));
+    v8 x;
+    asm volatile("s_load_dwordx8 %0, %1, 0x0\n\ts_waitcnt lgkmcnt(0)"
+                 : "=s"(x)
+                 : "s"(p), "v"(after));
+#pragma unroll
+    for (int l = 0; l < 4; ++l)
+      r[l] = __builtin_bit_cast(double, (unsigned long long)(unsigned)x[2 * l] |
+                                            ((unsigned long long)(unsigned)x[2 * l + 1] << 32));
+  } else {
+    typedef int v4 __attribute__((ext_vector_type(4)));
+    v4 x;
+    asm volatile("s_load_dwordx4 %0, %1, 0x0\n\ts_waitcnt lgkmcnt(0)"
+                 : "=s"(x)
+                 : "s"(p), "v"(after));
+#pragma unroll
+    for (int l = 0; l < 2; ++l)
+      r[l] = __builtin_bit_cast(double, (unsigned long long)(unsigned)x[2 * l] |
+                                            ((unsigned long long)(unsigned)x[2 * l + 1] << 32));
+  }
+}
+
+// out[k] = sum_l D(k,l) f[l] in the reference's order, one SGPR row at a time.
+template <int NQ, int ORDER, bool FMA, bool MAT_FIRST>
+__device__ __forceinline__ void sweep(const double *Drow, const double (&f)[NQ],
+                                      double (&out)[NQ]) {
+#pragma unroll
+  for (int k = 0; k < NQ; ++k) {
+    double r[NQ];
+    sgpr_row<NQ>(Drow, k, k ? out[k - 1] : f[0], r);
+    // x: Mat(i,l)*in(l); y,z: in(l)*Mat_tr(l,j) -- products commute
+    out[k] = contract<NQ, ORDER, FMA>([&](int l) { return r[l]; }, [&](int l) { return f[l]; });
+  }
+}
+
+template <int NQ, int ORDER, bool FMA, bool FACT_LIFT>
+__global__ void __launch_bounds__(64)
+    pencil_kernel(const StageParams P, const double *__restrict__ Drow,
+                  const double *__restrict__ Lift, const double *__restrict__ lfac) {
+  using S = PencilShape<NQ>;
+  constexpr int NP = S::NP, NFP = S::NFP, EPW = S::EPW, PAD = S::PAD, FPAD = S::FPAD;
+  extern __shared__ double smem[];
+  const int lane = threadIdx.x;
+  const int el = lane / NFP;
+  const int pl = lane - el * NFP;  // line index a + NQ*b
+  const int la = pl % NQ, lb = pl / NQ;
+  double *Bq = smem + el * S::PER_ELEM;
+  double *Bu = Bq + PAD;  // u, then D_x
+  double *Bv = Bu + PAD;  // v, then D_y
+  double *Bf = Bv + PAD;  // fluxes of faces -y, +y, -x, +x
+
+  // XCD-aware element order: workgroup b runs on XCD b % 8; XCD x walks the
+  // x-th eighth of the element groups in order.
+  const long long ngroups = (P.Ne + EPW - 1) / EPW;
+  const long long per = (ngroups + kNumXcd - 1) / kNumXcd;
+  const long long g = (long long)(blockIdx.x % kNumXcd) * per + blockIdx.x / kNumXcd;
+  if (g >= ngroups) return;  // uniform over the workgroup
+  const long long e = g * EPW + el;
+  const bool valid = e < P.Ne;
+  const long long eb = valid ? e : g * EPW;  // a valid element for address math
+
+  // ---- loads: own z-line (i,j) = (la,lb) and six neighbour face traces ----
+  const long long base = eb * NP + pl;
+  double q[NQ], u[NQ], v[NQ], w[NQ], q0[NQ];
+  const bool need_q0 = P.mode == kModeStageN;
+#pragma unroll
+  for (int k = 0; k < NQ; ++k) {
+    q[k] = P.qin[base + NFP * k];
+    u[k] = P.u[base + NFP * k];
+    v[k] = P.v[base + NFP * k];
+    w[k] = P.w[base + NFP * k];
+    q0[k] = need_q0 ? P.q0[base + NFP * k] : 0.0;
+  }
+  const int32_t *nb = P.nbr + eb * 6;
+  // z faces at (i,j): -z (face 4) sees the neighbour's k = NQ-1, +z (face 5)
+  // its k = 0
+  const long long a4 = (long long)nb[4] * NP + pl + NFP * (NQ - 1);
+  const long long a5 = (long long)nb[5] * NP + pl;
+  // y faces at (i,k) = (la,lb): -y (face 0) sees j = NQ-1, +y (face 2) j = 0
+  const long long a0 = (long long)nb[0] * NP + la + NQ * (NQ - 1) + NFP * lb;
+  const long long a2 = (long long)nb[2] * NP + la + NFP * lb;
+  // x faces at (j,k) = (la,lb): -x (face 3) sees i = NQ-1, +x (face 1) i = 0
+  const long long a3 = (long long)nb[3] * NP + (NQ - 1) + NQ * pl;
+  const long long a1 = (long long)nb[1] * NP + NQ * pl;
+  const double qP4 = P.qin[a4], wP4 = P.w[a4], qP5 = P.qin[a5], wP5 = P.w[a5];
+  const double qP0 = P.qin[a0], vP0 = P.v[a0], qP2 = P.qin[a2], vP2 = P.v[a2];
+  const double qP3 = P.qin[a3], uP3 = P.u[a3], qP1 = P.qin[a1], uP1 = P.u[a1];
+
+  // ---- z phase (registers) ---------------------------------------------
+  double dz[NQ];
+  {
+    double fz[NQ];
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) fz[k] = q[k] * w[k];
+    sweep<NQ, ORDER, FMA, false>(Drow, fz, dz);
+  }
+  const double flux4 = upwind_flux(q[0], qP4, -w[0], -wP4, P.fs[4]);
+  const double flux5 = upwind_flux(q[NQ - 1], qP5, w[NQ - 1], wP5, P.fs[5]);
+#pragma unroll
+  for (int k = 0; k < NQ; ++k) {
+    const int s = pidx<NQ>(pl + NFP * k);
+    Bq[s] = q[k];
+    Bu[s] = u[k];
+    Bv[s] = v[k];
+  }
+  __syncthreads();
+
+  // ---- y phase: line (i,k) = (la,lb) -------------------------------------
+  {
+    double qy[NQ], vy[NQ], fy[NQ], dy[NQ];
+#pragma unroll
+    for (int l = 0; l < NQ; ++l) {
+      const int s = pidx<NQ>(la + NQ * l + NFP * lb);
+      qy[l] = Bq[s];
+      vy[l] = Bv[s];
+      fy[l] = qy[l] * vy[l];
+    }
+    sweep<NQ, ORDER, FMA, false>(Drow, fy, dy);
+    Bf[0 * FPAD + pidx<NQ>(pl)] = upwind_flux(qy[0], qP0, -vy[0], -vP0, P.fs[0]);
+    Bf[1 * FPAD + pidx<NQ>(pl)] = upwind_flux(qy[NQ - 1], qP2, vy[NQ - 1], vP2, P.fs[2]);
+#pragma unroll
+    for (int j = 0; j < NQ; ++j) Bv[pidx<NQ>(la + NQ * j + NFP * lb)] = dy[j];
+  }
+
+  // ---- x phase: line (j,k) = (la,lb) -------------------------------------
+  {
+    double qx[NQ], ux[NQ], fx[NQ], dx[NQ];
+#pragma unroll
+    for (int l = 0; l < NQ; ++l) {
+      const int s = pidx<NQ>(l + NQ * pl);
+      qx[l] = Bq[s];
+      ux[l] = Bu[s];
+      fx[l] = qx[l] * ux[l];
+    }
+    sweep<NQ, ORDER, FMA, true>(Drow, fx, dx);
+    Bf[2 * FPAD + pidx<NQ>(pl)] = upwind_flux(qx[0], qP3, -ux[0], -uP3, P.fs[3]);
+    Bf[3 * FPAD + pidx<NQ>(pl)] = upwind_flux(qx[NQ - 1], qP1, ux[NQ - 1], uP1, P.fs[1]);
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) Bu[pidx<NQ>(i + NQ * pl)] = dx[i];
+  }
+  __syncthreads();
+
+  // ---- final z phase: lift, combine, RK ----------------------------------
+  if (!valid) return;
+  const int i = la, j = lb;
+  const double e1s = P.es[0], e2s = P.es[1], e3s = P.es[2];
+#pragma unroll
+  for (int k = 0; k < NQ; ++k) {
+    const int s = pidx<NQ>(pl + NFP * k);
+    const double dxv = Bu[s];
+    const double dyv = Bv[s];
+    const double f0 = Bf[0 * FPAD + pidx<NQ>(i + NQ * k)];  // in(i,k,1)
+    const double f2 = Bf[1 * FPAD + pidx<NQ>(i + NQ * k)];  // in(i,k,3)
+    const double f3 = Bf[2 * FPAD + pidx<NQ>(j + NQ * k)];  // in(j,k,4)
+    const double f1 = Bf[3 * FPAD + pidx<NQ>(j + NQ * k)];  // in(j,k,2)
+    double L[6];
+    if constexpr (FACT_LIFT) {
+      // l_-(normal index) for -y,-x,-z; l_+ for +x,+y,+z; lfac = [l_-, l_+]
+      L[0] = lfac[j];
+      L[1] = lfac[NQ + i];
+      L[2] = lfac[NQ + j];
+      L[3] = lfac[i];
+      L[4] = lfac[k];
+      L[5] = lfac[NQ + k];
+    } else {
+#pragma unroll
+      for (int f = 0; f < 6; ++f) L[f] = Lift[pl + NFP * k + NP * f];  // Lift(i,j,k,f)
+    }
+    // tensorprod_Lift_hexahedral order (mod_dg_optr_kernel.f90:277-283)
+    double lift = L[0] * f0;
+    lift = madd<FMA>(L[1], f1, lift);
+    lift = madd<FMA>(L[2], f2, lift);
+    lift = madd<FMA>(L[3], f3, lift);
+    lift = madd<FMA>(L[4], flux4, lift);
+    lift = madd<FMA>(L[5], flux5, lift);
+    // dqdt = -(Es1 Dx + Es2 Dy + Es3 Dz + Lift) (mod_advect3d_eq.f90:194-198)
+    double sum = e1s * dxv;
+    sum = madd<FMA>(e2s, dyv, sum);
+    sum = madd<FMA>(e3s, dz[k], sum);
+    const double dq = -(sum + lift);
+    const long long idx = base + NFP * k;
+    if (P.mode == kModeTend) {
+      P.dqdt[idx] = dq;
+    } else {
+      const double qq = q[k];
+      const double qs = P.mode == kModeStage1 ? qq : q0[k];
+      if (P.mode == kModeStage1) P.q0[idx] = qq;
+      double qn;
+      if constexpr (FMA) qn = __builtin_fma(P.a, qs, P.b * __builtin_fma(P.dt, dq, qq));
+      else qn = P.a * qs + P.b * (qq + P.dt * dq);
+      P.qout[idx] = qn;
+    }
+  }
+}
+
+}  // namespace dgk

@@ -23,7 +23,7 @@ OPERATOR_DIR = os.path.join(REPO, "operator_data")
 OK = 0
 ERR_ARG, ERR_OPTTYPE, ERR_ORDER, ERR_IO, ERR_HIP, ERR_NOMEM, ERR_STATE = range(1, 8)
 TYPE_GENERAL, TYPE_OPT1, TYPE_HIP = 1, 2, 3
-FLAG_FMA, FLAG_FORCE_TABLES = 0x1, 0x2
+FLAG_FMA, FLAG_FORCE_TABLES, FLAG_NODE_KERNEL, FLAG_FACTORED_LIFT = 0x1, 0x2, 0x4, 0x8
 FIELD_Q, FIELD_Q0, FIELD_DQDT, FIELD_U, FIELD_V, FIELD_W = range(6)
 
 EXPORTS = (
@@ -242,6 +242,10 @@ class Solver:
     @property
     def compressed(self):
         return bool(self.query(0))
+
+    @property
+    def kernel_kind(self):
+        return "pencil" if self.query(6) else "node"
 
     def time_stages(self, dt, nstage):
         ms = C.c_double(0.0)

@@ -74,7 +74,13 @@ def test_batched_operators_bit_exact(gpu, golden, p, tname):
 
 def _solver_variants(ne, p, D, L, typeid, flags=0):
     m = dgk.mesh_setup(*ne, p)
-    yield "structured", dgk.Solver.structured(*ne, p, D, L, typeid=typeid, flags=flags)
+    s = dgk.Solver.structured(*ne, p, D, L, typeid=typeid, flags=flags)
+    assert s.kernel_kind == ("pencil" if p + 1 in (2, 4, 8) else "node")
+    yield "structured-" + s.kernel_kind, s
+    if p + 1 in (2, 4, 8):
+        s = dgk.Solver.structured(*ne, p, D, L, typeid=typeid, flags=flags | dgk.FLAG_NODE_KERNEL)
+        assert s.kernel_kind == "node"
+        yield "structured-node", s
     args = (p + 1, m["Ne"], m["NeA"], D, L, m["VMapM"], m["VMapP"], m["normal_fn"],
             m["Escale"], m["Fscale"], m["halo_src_map"])
     s = dgk.Solver.from_tables(*args, typeid=typeid, flags=flags)
@@ -133,31 +139,41 @@ def test_trajectory_bit_exact(gpu, golden, case, tname):
         s.close()
 
 
+FMA_VARIANTS = [("fma", dgk.FLAG_FMA), ("fma-node", dgk.FLAG_FMA | dgk.FLAG_NODE_KERNEL),
+                ("fma-factored", dgk.FLAG_FMA | dgk.FLAG_FACTORED_LIFT),
+                ("factored", dgk.FLAG_FACTORED_LIFT)]
+
+
 @pytest.mark.parametrize("case", CS.RUN_CASES[:3], ids=lambda c: CS.case_key(*c[0], c[1], c[2]))
-def test_trajectory_fma_tolerance(gpu, case):
-    """FMA contractions: rel-L2 <= 1e-12 after one step, <= 1e-10 after nstep
-    against the oracle (OPT1 order)."""
+@pytest.mark.parametrize("variant", FMA_VARIANTS, ids=lambda v: v[0])
+def test_trajectory_fma_tolerance(gpu, case, variant):
+    """FMA contractions and/or the factored lift: rel-L2 <= 1e-12 after one
+    step, <= 1e-10 after nstep against the oracle (OPT1 order)."""
     ne, p, nstep, vel, kind = case
+    name, flags = variant
+    if (flags & dgk.FLAG_FACTORED_LIFT) and p + 1 not in (2, 4, 8):
+        pytest.skip("the factored lift exists in the pencil kernel only")
     _, D, L = dgk.read_operator(p)
     om = O.Mesh(*ne, p)
     q, u, v, w, dt = CS.run_inputs(ne, p, vel, kind, om.pos_en)
     for steps, tol in ((1, 1e-12), (nstep, 1e-10)):
-        say(f"[fma] {ne} p={p} steps={steps}")
+        say(f"[{name}] {ne} p={p} steps={steps}")
         F = oracle_fields(om, q, u, v, w)
         om.run(*F, dt, steps, O.OPT1)
-        s = dgk.Solver.structured(*ne, p, D, L, flags=dgk.FLAG_FMA)
+        s = dgk.Solver.structured(*ne, p, D, L, flags=flags)
         s.set_fields(q, u, v, w)
         s.run(dt, steps)
         err = rel_l2(s.get_field(), F[0][:, :om.Ne])
         assert err <= tol, err
-        assert err > 0.0  # the fused variant really is a different rounding
+        assert err > 0.0  # a different rounding from the reference's
         s.close()
 
 
-def test_permuted_element_numbering_uses_tables(gpu):
-    """A valid mesh whose elements are renumbered is not the structured
-    numbering, so the solver keeps the per-node tables; results equal the
-    oracle run on the same permuted tables bit for bit."""
+def test_permuted_element_numbering(gpu):
+    """A valid mesh whose elements are renumbered: the compressed form (a
+    6-neighbour table) accepts any numbering, and the per-node table path is
+    forced as well; both equal the oracle run on the same permuted tables bit
+    for bit."""
     ne, p, nstep = (3, 2, 2), 4, 4
     om = O.Mesh(*ne, p)
     Np, Ne, NeA = om.Np, om.Ne, om.NeA
@@ -186,10 +202,43 @@ def test_permuted_element_numbering_uses_tables(gpu):
     F = oracle_fields(om, qn, un, vn, wn)
     dt = 2e-3
     om.run(*F, dt, nstep, O.OPT1)
-    say("[permuted] tables path")
-    s = dgk.Solver.from_tables(p + 1, Ne, NeA, om.D1D, om.Lift, VM, VP, nrm, Es, Fs, halo)
+    for flags, compressed in ((0, True), (dgk.FLAG_FORCE_TABLES, False)):
+        say(f"[permuted] flags={flags}")
+        s = dgk.Solver.from_tables(p + 1, Ne, NeA, om.D1D, om.Lift, VM, VP, nrm, Es, Fs, halo,
+                                   flags=flags)
+        assert s.compressed == compressed
+        s.set_fields(qn, un, vn, wn)
+        s.run(dt, nstep)
+        assert np.array_equal(s.get_field(), F[0][:, :Ne])
+        s.close()
+
+
+def test_stretched_mesh_keeps_tables(gpu):
+    """Element-dependent Escale/Fscale (a mesh stretched along x) cannot be
+    compressed to per-axis constants: the per-node tables are used and the
+    result equals the oracle's on the same tables."""
+    ne, p, nstep = (4, 2, 3), 3, 3
+    om = O.Mesh(*ne, p)
+    Ne = om.Ne
+    ix = np.arange(Ne) % ne[0]
+    widths = np.array([0.1, 0.2, 0.3, 0.4])  # element widths along x, sum 1
+    Es = om.Escale.copy(order="F")
+    Es[:, :, 0] = (2.0 / widths[ix])[None, :]
+    Fs = om.Fscale.copy(order="F")
+    Nfp = om.Nfp
+    for f in (1, 3):  # +x, -x faces
+        Fs[f * Nfp:(f + 1) * Nfp, :] = (2.0 / widths[ix])[None, :]
+    om.Escale, om.Fscale = Es, Fs
+    q = CS.uniform(91, (om.Np, Ne))
+    vel = [np.full((om.Np, Ne), c, order="F") for c in (0.6, -0.3, 0.2)]
+    F = oracle_fields(om, q, *vel)
+    dt = 1e-3
+    om.run(*F, dt, nstep, O.OPT1)
+    say("[stretched] tables path")
+    s = dgk.Solver.from_tables(p + 1, Ne, om.NeA, om.D1D, om.Lift, om.VMapM, om.VMapP,
+                               om.normal_fn, Es, Fs, om.halo_src_map)
     assert not s.compressed
-    s.set_fields(qn, un, vn, wn)
+    s.set_fields(q, *vel)
     s.run(dt, nstep)
     assert np.array_equal(s.get_field(), F[0][:, :Ne])
     s.close()
