@@ -13,7 +13,12 @@ modules (oracle/_ref, compiled from source) timed on the host cores over a
 bounded sample of the workload.
 
     python bench.py [--steps K] [--warmup W] [--workload c3_64cube_p7]
-                    [--variant exact|fma] [--no-cpu-baseline]
+                    [--variant factored|exact|fma|...] [--no-cpu-baseline]
+                    [--copy-calibration]
+
+--copy-calibration first copies 1 GiB device to device (hipMemcpy's copy
+kernel) three times, so that a rocprofv3 --pmc run of this script also
+records what FETCH_SIZE/WRITE_SIZE read for a copy of known size.
 """
 from __future__ import annotations
 
@@ -37,6 +42,7 @@ BYTES_PER_DOF = 48  # per DOF-update and stage, per-node FP64 velocity
 PROFILES = os.path.join(REPO, "profiles")
 # solver variants: flags of dgk_create_structured
 VARIANTS = {
+    "factored": dgk.FLAG_FACTORED_LIFT,                # FP64, lift by the 1D vectors
     "exact": 0,                                        # bit-identical to the reference
     "fma": dgk.FLAG_FMA,                               # fused multiply-adds
     "fma_factored": dgk.FLAG_FMA | dgk.FLAG_FACTORED_LIFT,
@@ -111,15 +117,26 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="c3_64cube_p7", choices=sorted(ic.WORKLOADS))
-    ap.add_argument("--variant", default="exact", choices=sorted(VARIANTS))
+    ap.add_argument("--variant", default="factored", choices=sorted(VARIANTS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-steps", type=int, default=5)
+    ap.add_argument("--copy-calibration", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
               "sharding of the mesh is a design, not a measured path", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_calibration:
+        import torch
+        a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+        log("copied 1 GiB device-to-device 3 times (counter calibration)")
 
     spec = ic.WORKLOADS[args.workload]
     ne, p, vel = spec["ne"], spec["p"], spec["vel"]
@@ -162,6 +179,14 @@ def main():
     achieved = bytes_launch / (ms_stage * 1e-3) / 1e9
     qf = s.get_field()
     finite = bool(np.isfinite(qf).all())
+    # the bit-identical variant's stage kernel, timed on the same state
+    ms_exact = None
+    if args.variant != "exact":
+        se = dgk.Solver.structured(*ne, p, D, L, flags=VARIANTS["exact"])
+        se.set_fields(qf, *(np.full_like(qf, c) for c in vel))
+        se.run(dt, 1)
+        ms_exact = se.time_stages(dt, nstage)
+        se.close()
     # the sine-mode field has (near) zero mean: drift is measured against the
     # integral of |q|
     drift = abs(mass(qf, w3, J)[0] - mass0) / mass_abs
@@ -198,6 +223,10 @@ def main():
                      "algorithmic_bytes_per_launch": bytes_launch},
         "cpu_baseline": cpu,
         "checks": {"finite": finite, "mass_drift_rel": drift},
+        "exact_variant": None if ms_exact is None else {
+            "kernel_us": round(ms_exact * 1e3, 2),
+            "gdof_updates_per_s": round(dof / (ms_exact * 1e-3) / 1e9, 2),
+            "hbm_frac": round(bytes_launch / (ms_exact * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
     }
     print(json.dumps(out), flush=True)
 

@@ -142,8 +142,14 @@ int dgk_run(dgk_solver *s, double dt, int nstep, double *minmax);
 int dgk_synchronize(dgk_solver *s);
 /* what: 0 = connectivity mode (1 compressed, 0 tables), 1 = Ne, 2 = Np,
  * 3 = NeA, 4 = grid size of the stage kernel, 5 = block size, 6 = stage
- * kernel kind (0 thread-per-node, 1 pencil), 7 = LDS bytes per workgroup. */
+ * kernel kind (0 thread-per-node, 1 pencil), 7 = LDS bytes per workgroup,
+ * 8 = tuning variant of the stage kernel. */
 int dgk_query(dgk_solver *s, int what, int64_t *value);
+/* Tuning variants of the stage kernel (code shapes with identical results):
+ * *n receives their number; dgk_set_variant selects one (-1 = the built-in
+ * default for this Nq).  Used by the autotuner. */
+int dgk_num_variants(dgk_solver *s, int *n);
+int dgk_set_variant(dgk_solver *s, int variant);
 /* Times `nstage` consecutive stage launches with HIP events on the solver's
  * stream; returns the mean milliseconds per stage launch. */
 int dgk_time_stages(dgk_solver *s, double dt, int nstage, double *ms_per_stage);

@@ -30,6 +30,7 @@ using StageFn = void (*)(const StageParams);
 struct HostOps {
   const double *Drow;
   const double *lfac;
+  const double *LiftP;  // pair-packed Lift for the pencil kernel
 };
 
 struct StageKernel;
@@ -43,6 +44,7 @@ struct StageKernel {
   size_t lds = 0;
   int epb = 1;       // elements per workgroup (node kernel) or wavefront (pencil)
   bool pencil = false;
+  int variant = 0;   // tuning variant (pencil kernel)
   long long grid = 0;
 };
 
@@ -64,40 +66,51 @@ StageKernel make_stage() {
   return k;
 }
 
-template <int NQ, int ORDER, bool FMA, bool FACT>
+template <int NQ, int ORDER, bool FMA, bool FACT, int VAR>
 void launch_pencil(const StageKernel &k, const StageParams &P, const HostOps &H,
                    const double *Lift, hipStream_t st) {
-  hipLaunchKernelGGL((pencil_kernel<NQ, ORDER, FMA, FACT>), dim3((unsigned)k.grid), dim3(64),
-                     k.lds, st, P, H.Drow, Lift, H.lfac);
+  hipLaunchKernelGGL((pencil_kernel<NQ, ORDER, FMA, FACT, VAR>), dim3((unsigned)k.grid),
+                     dim3(64), k.lds, st, P, H.Drow, H.LiftP, H.lfac);
 }
 
-template <int NQ, int ORDER, bool FMA, bool FACT>
+template <int NQ, int ORDER, bool FMA, bool FACT, int VAR>
 StageKernel make_pencil() {
   StageKernel k;
-  k.kernel = (const void *)pencil_kernel<NQ, ORDER, FMA, FACT>;
-  k.launch = launch_pencil<NQ, ORDER, FMA, FACT>;
+  k.kernel = (const void *)pencil_kernel<NQ, ORDER, FMA, FACT, VAR>;
+  k.launch = launch_pencil<NQ, ORDER, FMA, FACT, VAR>;
   k.nt = 64;
   k.lds = PencilShape<NQ>::LDS;
   k.epb = PencilShape<NQ>::EPW;
   k.pencil = true;
+  k.variant = VAR;
   return k;
 }
 
-template <int NQ>
-StageKernel pick_pencil_nq(int order, bool fma, bool fact) {
-  if (order == kGeneral) {
-    if (fma) return fact ? make_pencil<NQ, kGeneral, true, true>() : make_pencil<NQ, kGeneral, true, false>();
-    return fact ? make_pencil<NQ, kGeneral, false, true>() : make_pencil<NQ, kGeneral, false, false>();
+template <int NQ, int ORDER, bool FMA, bool FACT>
+StageKernel pick_pencil_var(int var) {
+  switch (var) {
+  case 0: return make_pencil<NQ, ORDER, FMA, FACT, 0>();
+  case 1: return make_pencil<NQ, ORDER, FMA, FACT, 1>();
+  case 2: return make_pencil<NQ, ORDER, FMA, FACT, 2>();
+  default: return make_pencil<NQ, ORDER, FMA, FACT, 3>();
   }
-  if (fma) return fact ? make_pencil<NQ, kOpt1, true, true>() : make_pencil<NQ, kOpt1, true, false>();
-  return fact ? make_pencil<NQ, kOpt1, false, true>() : make_pencil<NQ, kOpt1, false, false>();
 }
 
-StageKernel pick_pencil(int Nq, int order, bool fma, bool fact) {
+template <int NQ>
+StageKernel pick_pencil_nq(int order, bool fma, bool fact, int var) {
+  if (order == kGeneral) {
+    if (fma) return fact ? pick_pencil_var<NQ, kGeneral, true, true>(var) : pick_pencil_var<NQ, kGeneral, true, false>(var);
+    return fact ? pick_pencil_var<NQ, kGeneral, false, true>(var) : pick_pencil_var<NQ, kGeneral, false, false>(var);
+  }
+  if (fma) return fact ? pick_pencil_var<NQ, kOpt1, true, true>(var) : pick_pencil_var<NQ, kOpt1, true, false>(var);
+  return fact ? pick_pencil_var<NQ, kOpt1, false, true>(var) : pick_pencil_var<NQ, kOpt1, false, false>(var);
+}
+
+StageKernel pick_pencil(int Nq, int order, bool fma, bool fact, int var) {
   switch (Nq) {
-  case 2: return pick_pencil_nq<2>(order, fma, fact);
-  case 4: return pick_pencil_nq<4>(order, fma, fact);
-  case 8: return pick_pencil_nq<8>(order, fma, fact);
+  case 2: return pick_pencil_nq<2>(order, fma, fact, var);
+  case 4: return pick_pencil_nq<4>(order, fma, fact, var);
+  case 8: return pick_pencil_nq<8>(order, fma, fact, var);
   default: return StageKernel{};
   }
 }
@@ -166,10 +179,11 @@ struct dgk_solver {
   bool fma = false;
   bool structured = false;
   int flags = 0;
+  int variant = -1;  // pencil tuning variant; -1 = built-in default
   HostOps ops;
   double *q[2] = {nullptr, nullptr};
   double *q0 = nullptr, *u = nullptr, *v = nullptr, *w = nullptr, *dqdt = nullptr;
-  double *D = nullptr, *Lift = nullptr, *lfac = nullptr, *Drow = nullptr;
+  double *D = nullptr, *Lift = nullptr, *lfac = nullptr, *Drow = nullptr, *LiftP = nullptr;
   int32_t *nbr = nullptr, *vmapM = nullptr, *vmapP = nullptr, *halo = nullptr;
   double *nrm = nullptr, *fscale = nullptr, *escale = nullptr;
   double fs[6] = {0, 0, 0, 0, 0, 0}, es[3] = {0, 0, 0};
@@ -185,11 +199,19 @@ struct dgk_solver {
 
 namespace {
 
+// Built-in default tuning variant per Nq (overridden by dgk_set_variant):
+// non-temporal streaming stores measured fastest at Nq = 8 (64^3 elements).
+int default_pencil_variant(int Nq) {
+  (void)Nq;
+  return kPencilNtStore;
+}
+
 int configure_launch(dgk_solver *s) {
   const bool use_pencil = s->structured && !(s->flags & DGK_FLAG_NODE_KERNEL) &&
                           (s->Nq == 2 || s->Nq == 4 || s->Nq == 8);
   if (use_pencil)
-    s->sk = pick_pencil(s->Nq, s->order, s->fma, (s->flags & DGK_FLAG_FACTORED_LIFT) != 0);
+    s->sk = pick_pencil(s->Nq, s->order, s->fma, (s->flags & DGK_FLAG_FACTORED_LIFT) != 0,
+                        s->variant >= 0 ? s->variant : default_pencil_variant(s->Nq));
   else
     s->sk = pick_stage(s->Nq, s->order, s->fma, s->structured);
   if (!s->sk.kernel) return fail(DGK_ERR_ORDER, "Nq %d not supported", s->Nq);
@@ -341,8 +363,23 @@ int finish_create(dgk_solver *s, const double *D1D, const double *Lift) {
   const int Nq = s->Nq, Np = s->Np;
   const size_t nq2 = (size_t)Nq * Nq, nlift = (size_t)6 * Np;
   if ((rc = dalloc(&s->D, nq2)) || (rc = dalloc(&s->Lift, nlift)) ||
-      (rc = dalloc(&s->lfac, 2 * (size_t)Nq)) || (rc = dalloc(&s->Drow, nq2)))
+      (rc = dalloc(&s->lfac, 2 * (size_t)Nq)) || (rc = dalloc(&s->Drow, nq2)) ||
+      (rc = dalloc(&s->LiftP, nlift)))
     return rc;
+  // pair-packed Lift for the pencil kernel: line pl = i + Nq*j holds the
+  // 6*Nq values v = 6k + f, Lift(i,j,k,f); pair m of line pl at
+  // [(m*Nq^2 + pl)*2 + {0,1}]
+  {
+    const int Nfp = Nq * Nq;
+    std::vector<double> lp(nlift);
+    for (int pl = 0; pl < Nfp; ++pl)
+      for (int v = 0; v < 6 * Nq; ++v) {
+        const int k = v / 6, f = v % 6, m = v / 2;
+        lp[((size_t)m * Nfp + pl) * 2 + (v & 1)] = Lift[pl + (size_t)Nfp * k + (size_t)Np * f];
+      }
+    HIPOK(hipMemcpy(s->LiftP, lp.data(), nlift * sizeof(double), hipMemcpyHostToDevice));
+    s->ops.LiftP = s->LiftP;
+  }
   HIPOK(hipMemcpy(s->D, D1D, nq2 * sizeof(double), hipMemcpyHostToDevice));
   HIPOK(hipMemcpy(s->Lift, Lift, nlift * sizeof(double), hipMemcpyHostToDevice));
   std::vector<double> dr(nq2);
@@ -379,7 +416,7 @@ extern "C" {
 int dgk_destroy(dgk_solver *s) {
   if (!s) return DGK_OK;
   drop_graphs(s);
-  void *bufs[] = {s->q[0], s->q[1], s->q0, s->u, s->v, s->w, s->dqdt, s->D, s->Lift, s->lfac, s->Drow,
+  void *bufs[] = {s->q[0], s->q[1], s->q0, s->u, s->v, s->w, s->dqdt, s->D, s->Lift, s->lfac, s->Drow, s->LiftP,
                   s->nbr, s->vmapM, s->vmapP, s->halo, s->nrm, s->fscale, s->escale,
                   s->part, s->mm};
   for (void *b : bufs)
@@ -643,9 +680,29 @@ int dgk_query(dgk_solver *s, int what, int64_t *value) {
   case 5: *value = s->sk.nt; break;
   case 6: *value = s->sk.pencil ? 1 : 0; break;
   case 7: *value = (int64_t)s->sk.lds; break;
+  case 8: *value = s->sk.variant; break;
   default: return fail(DGK_ERR_ARG, "unknown query %d", what);
   }
   return DGK_OK;
+}
+
+int dgk_num_variants(dgk_solver *s, int *n) {
+  if (!s || !n) return fail(DGK_ERR_ARG, "null argument");
+  *n = s->sk.pencil ? kPencilVariants : 1;
+  return DGK_OK;
+}
+
+int dgk_set_variant(dgk_solver *s, int variant) {
+  if (!s) return fail(DGK_ERR_ARG, "null solver");
+  if (!s->sk.pencil) {
+    if (variant != 0) return fail(DGK_ERR_ARG, "this stage kernel has one variant");
+    return DGK_OK;
+  }
+  if (variant < -1 || variant >= kPencilVariants)
+    return fail(DGK_ERR_ARG, "variant %d outside -1..%d", variant, kPencilVariants - 1);
+  s->variant = variant;
+  drop_graphs(s);
+  return configure_launch(s);
 }
 
 int dgk_time_stages(dgk_solver *s, double dt, int nstage, double *ms_per_stage) {

@@ -100,10 +100,28 @@ __device__ __forceinline__ void sweep(const double *Drow, const double (&f)[NQ],
   }
 }
 
-template <int NQ, int ORDER, bool FMA, bool FACT_LIFT>
+// Tuning variants of the pencil kernel (bit set):
+//   kPencilLiftEarly  issue the dense Lift loads before the y and x phases so
+//                     that their latency overlaps those phases (costs VGPRs)
+//   kPencilNtStore    write q, q0 and dqdt with non-temporal stores (no reuse
+//                     within a launch)
+constexpr int kPencilLiftEarly = 1;
+constexpr int kPencilNtStore = 2;
+constexpr int kPencilVariants = 4;
+
+template <bool NT>
+__device__ __forceinline__ void st(double *p, double v) {
+  if constexpr (NT) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+
+template <int NQ, int ORDER, bool FMA, bool FACT_LIFT, int VAR>
 __global__ void __launch_bounds__(64)
     pencil_kernel(const StageParams P, const double *__restrict__ Drow,
                   const double *__restrict__ Lift, const double *__restrict__ lfac) {
+  // Lift: the pair-packed copy described at load_lift below
+  constexpr bool LIFT_EARLY = (VAR & kPencilLiftEarly) && !FACT_LIFT;
+  constexpr bool NT = (VAR & kPencilNtStore) != 0;
   using S = PencilShape<NQ>;
   constexpr int NP = S::NP, NFP = S::NFP, EPW = S::EPW, PAD = S::PAD, FPAD = S::FPAD;
   extern __shared__ double smem[];
@@ -170,6 +188,20 @@ __global__ void __launch_bounds__(64)
     Bu[s] = u[k];
     Bv[s] = v[k];
   }
+  // Dense lift coefficients of this lane's z-line, Lift(i,j,k,f) for the
+  // 6*NQ pairs (k,f), packed by the host as 16-byte pairs that the lanes of a
+  // wavefront read contiguously: Lift[(m*NFP + line)*2 + {0,1}] holds
+  // values 2m and 2m+1 of the line, value v = 6k + f.
+  const double2 *LP = reinterpret_cast<const double2 *>(Lift);
+  double Le[LIFT_EARLY ? 6 * NQ : 1];
+  if constexpr (LIFT_EARLY) {
+#pragma unroll
+    for (int m = 0; m < 3 * NQ; ++m) {
+      const double2 pr = LP[m * NFP + pl];
+      Le[2 * m] = pr.x;
+      Le[2 * m + 1] = pr.y;
+    }
+  }
   __syncthreads();
 
   // ---- y phase: line (i,k) = (la,lb) -------------------------------------
@@ -229,9 +261,16 @@ __global__ void __launch_bounds__(64)
       L[3] = lfac[i];
       L[4] = lfac[k];
       L[5] = lfac[NQ + k];
+    } else if constexpr (LIFT_EARLY) {
+#pragma unroll
+      for (int f = 0; f < 6; ++f) L[f] = Le[6 * k + f];  // Lift(i,j,k,f)
     } else {
 #pragma unroll
-      for (int f = 0; f < 6; ++f) L[f] = Lift[pl + NFP * k + NP * f];  // Lift(i,j,k,f)
+      for (int m = 0; m < 3; ++m) {  // values 6k..6k+5 = pairs 3k..3k+2
+        const double2 pr = LP[(3 * k + m) * NFP + pl];
+        L[2 * m] = pr.x;
+        L[2 * m + 1] = pr.y;
+      }
     }
     // tensorprod_Lift_hexahedral order (mod_dg_optr_kernel.f90:277-283)
     double lift = L[0] * f0;
@@ -247,15 +286,15 @@ __global__ void __launch_bounds__(64)
     const double dq = -(sum + lift);
     const long long idx = base + NFP * k;
     if (P.mode == kModeTend) {
-      P.dqdt[idx] = dq;
+      st<NT>(P.dqdt + idx, dq);
     } else {
       const double qq = q[k];
       const double qs = P.mode == kModeStage1 ? qq : q0[k];
-      if (P.mode == kModeStage1) P.q0[idx] = qq;
+      if (P.mode == kModeStage1) st<NT>(P.q0 + idx, qq);
       double qn;
       if constexpr (FMA) qn = __builtin_fma(P.a, qs, P.b * __builtin_fma(P.dt, dq, qq));
       else qn = P.a * qs + P.b * (qq + P.dt * dq);
-      P.qout[idx] = qn;
+      st<NT>(P.qout + idx, qn);
     }
   }
 }

@@ -31,7 +31,7 @@ EXPORTS = (
     "dgk_mesh_sizes", "dgk_mesh_setup", "dgk_create", "dgk_create_structured",
     "dgk_destroy", "dgk_set_fields", "dgk_get_field", "dgk_field_ptr", "dgk_cal_tend",
     "dgk_rk_stage", "dgk_step", "dgk_run", "dgk_synchronize", "dgk_query",
-    "dgk_time_stages", "dgk_advect3d_eq_cal_tend", "dgk_divlike_dirXYZ_batched",
+    "dgk_time_stages", "dgk_num_variants", "dgk_set_variant", "dgk_advect3d_eq_cal_tend", "dgk_divlike_dirXYZ_batched",
     "dgk_lift_hexahedral_batched",
 )
 
@@ -76,6 +76,8 @@ def lib():
         "dgk_synchronize": ([vp], i),
         "dgk_query": ([vp, i, vp], i),
         "dgk_time_stages": ([vp, d, i, vp], i),
+        "dgk_num_variants": ([vp, vp], i),
+        "dgk_set_variant": ([vp, i], i),
         "dgk_advect3d_eq_cal_tend": ([vp] * 13 + [i] * 6, i),
         "dgk_divlike_dirXYZ_batched": ([i, i, i] + [vp] * 9, i),
         "dgk_lift_hexahedral_batched": ([i, i, vp, vp, vp, vp], i),
@@ -246,6 +248,19 @@ class Solver:
     @property
     def kernel_kind(self):
         return "pencil" if self.query(6) else "node"
+
+    @property
+    def num_variants(self):
+        n = C.c_int(0)
+        _check(lib().dgk_num_variants(self._h, C.byref(n)))
+        return n.value
+
+    @property
+    def variant(self):
+        return self.query(8)
+
+    def set_variant(self, v):
+        _check(lib().dgk_set_variant(self._h, int(v)))
 
     def time_stages(self, dt, nstage):
         ms = C.c_double(0.0)

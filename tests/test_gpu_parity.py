@@ -244,6 +244,28 @@ def test_stretched_mesh_keeps_tables(gpu):
     s.close()
 
 
+@pytest.mark.parametrize("p", [1, 3, 7])
+def test_every_tuning_variant_is_bit_exact(gpu, golden, p):
+    """All code shapes the autotuner may pick give the reference's bits."""
+    case = [c for c in CS.RUN_CASES if c[1] == p] or [((4, 3, 2), p, 3, (0.7, 0.2, -0.5), "rand")]
+    ne, p, nstep, vel, kind = case[0]
+    x1D, D, L = dgk.read_operator(p)
+    m = dgk.mesh_setup(*ne, p, x1D=x1D, tables=False)
+    q, u, v, w, dt = CS.run_inputs(ne, p, vel, kind, m["pos_en"])
+    om = O.Mesh(*ne, p)
+    F = oracle_fields(om, q, u, v, w)
+    om.run(*F, dt, nstep, O.OPT1)
+    s = dgk.Solver.structured(*ne, p, D, L)
+    for var in range(s.num_variants):
+        say(f"[variant] p={p} variant={var}")
+        s.set_variant(var)
+        assert s.variant == var
+        s.set_fields(q, u, v, w)
+        s.run(dt, nstep)
+        assert np.array_equal(s.get_field(), F[0][:, :om.Ne]), var
+    s.close()
+
+
 def test_graph_step_equals_stagewise(gpu):
     """Replays of the captured time step, with eager work and host syncs in
     between, equal the stage-by-stage launches bit for bit."""
