@@ -99,5 +99,30 @@ def run_inputs(ne, p, vel, kind, pos):
     return q, u, v, w, dt
 
 
+# The reference program's own configuration file knobs for its stock run
+# (NeX=NeY=NeZ=4, PolyOrder=7, OPT1, dt=1e-3, nstep=1000, output every 100)
+# and a second, GENERAL-order case with a non-unit velocity.
+STOCK_RUNS = {
+    "stock_4cube_p7_opt1": dict(NeX=4, NeY=4, NeZ=4, PolyOrder=7, opt="OPT1", dt="1.0D-3",
+                                nstep=1000, output_interval=100, vel=None),
+    "3x5x2_p4_general": dict(NeX=3, NeY=5, NeZ=2, PolyOrder=4, opt="GENERAL", dt="2.0D-3",
+                             nstep=60, output_interval=20, vel=("0.6D0", "-1.1D0", "0.35D0")),
+}
+
+
+def namelist_text(cfg, opt=None):
+    """&PARAM_ADVECT3D text for a STOCK_RUNS entry (main.f90:103-107)."""
+    lines = ["&PARAM_ADVECT3D",
+             f"  NeX = {cfg['NeX']}, NeY = {cfg['NeY']}, NeZ = {cfg['NeZ']},",
+             f"  PolyOrder = {cfg['PolyOrder']},",
+             f'  DGOptrKernel_OptType = "{opt or cfg["opt"]}"',
+             f"  dt = {cfg['dt']},",
+             f"  nstep = {cfg['nstep']}, output_interval = {cfg['output_interval']},"]
+    if cfg["vel"]:
+        lines.append(f"  vel_x = {cfg['vel'][0]}, vel_y = {cfg['vel'][1]}, vel_z = {cfg['vel'][2]},")
+    lines.append("/")
+    return "\n".join(lines) + "\n"
+
+
 def case_key(*parts):
     return "_".join(str(x).replace(" ", "") for x in parts)

@@ -11,6 +11,7 @@ from __future__ import annotations
 
 import json
 import os
+import subprocess
 import sys
 
 import numpy as np
@@ -65,6 +66,18 @@ def main():
             out["run"][CS.case_key(*ne, p, nstep, t)] = {
                 "q": summary(qf), "dt": float(dt).hex(),
                 "minmax": [[float(a).hex() for a in col] for col in mm.T]}
+    # the reference program itself (main.f90 unmodified): its istep/min/max
+    # lines (main.f90:76-79) for the configurations of CS.STOCK_RUNS
+    out["stock"] = {}
+    prog = os.path.join(REPO, "oracle", "_ref", "scale-dg_extraction")
+    for name, cfg in CS.STOCK_RUNS.items():
+        d = O._workdir()
+        with open(os.path.join(d, "input.conf"), "w") as f:
+            f.write(CS.namelist_text(cfg))
+        r = subprocess.run([prog, "input.conf"], cwd=d, capture_output=True, text=True,
+                           check=True, env=dict(os.environ, OMP_NUM_THREADS="4"))
+        lines = [l for l in r.stdout.splitlines() if l[:8].strip().isdigit()]
+        out["stock"][name] = {"lines": lines}
     with open(CS.GOLDEN_JSON, "w") as f:
         json.dump(out, f, indent=1, sort_keys=True)
     print("wrote", CS.GOLDEN_JSON)

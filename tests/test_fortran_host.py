@@ -1,0 +1,93 @@
+"""The Fortran host (scale-dg-kernel-extraction_amd/host/dg_advect3d.f90,
+bound to libdgk.so through mod_dgk_hip) reading the reference's
+&PARAM_ADVECT3D namelist.
+
+On the GPU its istep/min/max lines must equal, character for character, the
+lines the reference program itself (main.f90, compiled from source) printed
+for the same configuration (tests/golden/golden.json "stock").  The CPU
+tests cover its error handling, which runs before any device call."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import cases as CS
+import oracle as O
+from conftest import REPO, say
+
+HOST = os.path.join(REPO, "scale-dg-kernel-extraction_amd", "lib", "dg_advect3d")
+OPDIR = os.path.join(REPO, "operator_data")
+
+pytestmark = pytest.mark.skipif(not os.path.exists(HOST), reason="Fortran host not built")
+
+
+def run_host(tmp_path, text, extra=""):
+    conf = tmp_path / "input.conf"
+    conf.write_text(text + extra)
+    return subprocess.run([HOST, str(conf)], cwd=tmp_path, capture_output=True, text=True,
+                          timeout=300)
+
+
+def hip_group(**kw):
+    items = [f'operator_data_dir = "{OPDIR}"'] + [f"{k} = {v}" for k, v in kw.items()]
+    return "&PARAM_DGK_HIP\n  " + ",\n  ".join(items) + "\n/\n"
+
+
+def test_requires_a_configuration_argument():
+    r = subprocess.run([HOST], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0 and "configuration file argument is required" in r.stderr
+
+
+@pytest.mark.parametrize("opt,msg", [("OPT1", "runs DGOptrKernel_OptType"),
+                                     ("CUDA", "Unsupported dgoptr_evaltype")])
+def test_rejects_other_operator_types(tmp_path, opt, msg):
+    cfg = CS.STOCK_RUNS["stock_4cube_p7_opt1"]
+    r = run_host(tmp_path, CS.namelist_text(cfg, opt=opt), hip_group())
+    assert r.returncode != 0
+    assert msg in r.stderr
+
+
+def test_missing_operator_data_is_reported(tmp_path):
+    cfg = dict(CS.STOCK_RUNS["stock_4cube_p7_opt1"])
+    r = run_host(tmp_path, CS.namelist_text(cfg, opt="HIP"),
+                 '&PARAM_DGK_HIP\n  operator_data_dir = "/nonexistent"\n/\n')
+    assert r.returncode != 0 and "dgk_read_operator failed" in r.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", sorted(CS.STOCK_RUNS))
+@pytest.mark.parametrize("connectivity", ["STRUCTURED", "TABLES"])
+def test_stdout_equals_reference_program(tmp_path, golden, name, connectivity):
+    cfg = CS.STOCK_RUNS[name]
+    say(f"[fortran host] {name} {connectivity}")
+    r = run_host(tmp_path, CS.namelist_text(cfg, opt="HIP"),
+                 hip_group(connectivity=f'"{connectivity}"', sum_order=f'"{cfg["opt"]}"'))
+    assert r.returncode == 0, r.stderr
+    lines = [l for l in r.stdout.splitlines() if l[:8].strip().isdigit()]
+    assert lines == golden["stock"][name]["lines"]
+
+
+@pytest.mark.gpu
+def test_final_field_equals_oracle(tmp_path):
+    """The whole final state, not only min/max: the host dumps its initial
+    and final owned q; the oracle run from the same initial q must end on
+    the same bits."""
+    cfg = dict(CS.STOCK_RUNS["3x5x2_p4_general"])
+    say("[fortran host] final field vs oracle")
+    r = run_host(tmp_path, CS.namelist_text(cfg, opt="HIP"),
+                 hip_group(sum_order='"GENERAL"', dump_initial='"q_init.bin"',
+                           dump_final='"q_final.bin"'))
+    assert r.returncode == 0, r.stderr
+    m = O.Mesh(cfg["NeX"], cfg["NeY"], cfg["NeZ"], cfg["PolyOrder"])
+    q0 = np.fromfile(tmp_path / "q_init.bin").reshape((m.Np, m.Ne), order="F")
+    qf = np.fromfile(tmp_path / "q_final.bin").reshape((m.Np, m.Ne), order="F")
+    vel = [float(x.replace("D", "e")) for x in cfg["vel"]]
+    F = []
+    for a in (q0, *(np.full_like(q0, c) for c in vel)):
+        f = m.field()
+        f[:, :m.Ne] = a
+        m.update_halo(f)
+        F.append(f)
+    m.run(*F, float(cfg["dt"].replace("D", "e")), cfg["nstep"], O.GENERAL)
+    assert np.array_equal(qf, F[0][:, :m.Ne])
