@@ -143,7 +143,8 @@ int dgk_synchronize(dgk_solver *s);
 /* what: 0 = connectivity mode (1 compressed, 0 tables), 1 = Ne, 2 = Np,
  * 3 = NeA, 4 = grid size of the stage kernel, 5 = block size, 6 = stage
  * kernel kind (0 thread-per-node, 1 pencil), 7 = LDS bytes per workgroup,
- * 8 = tuning variant of the stage kernel. */
+ * 8 = tuning variant of the stage kernel, 9 = tiled element order (1) or
+ * linear (0). */
 int dgk_query(dgk_solver *s, int what, int64_t *value);
 /* Tuning variants of the stage kernel (code shapes with identical results):
  * *n receives their number; dgk_set_variant selects one (-1 = the built-in

@@ -180,6 +180,7 @@ struct dgk_solver {
   bool structured = false;
   int flags = 0;
   int variant = -1;  // pencil tuning variant; -1 = built-in default
+  int NeX = 0, NeY = 0, NeZ = 0;  // structured extents in the reference numbering (0: unknown)
   HostOps ops;
   double *q[2] = {nullptr, nullptr};
   double *q0 = nullptr, *u = nullptr, *v = nullptr, *w = nullptr, *dqdt = nullptr;
@@ -199,6 +200,22 @@ struct dgk_solver {
 
 namespace {
 
+// Tiled element order of the pencil kernel (tiled_element): on for a known
+// structured numbering with at least one z-plane per XCD, unless the tuning
+// variant asks for the linear order.
+bool tiled_order(const dgk_solver *s) {
+  return s->sk.pencil && s->NeX > 0 && s->NeZ >= kNumXcd &&
+         !((s->variant >= 0 ? s->variant : 0) & kPencilLinearOrder);
+}
+
+void tile_shape(const dgk_solver *s, int *tx, int *ty, long long *tseq) {
+  *tx = std::min(16, s->NeX);
+  *ty = std::min(8, s->NeY);
+  const long long ntx = (s->NeX + *tx - 1) / *tx, nty = (s->NeY + *ty - 1) / *ty;
+  const long long nk = (s->NeZ + kNumXcd - 1) / kNumXcd;
+  *tseq = ntx * nty * (long long)(*tx) * (*ty) * nk;
+}
+
 // Built-in default tuning variant per Nq (overridden by dgk_set_variant):
 // non-temporal streaming stores measured fastest at Nq = 8 (64^3 elements).
 int default_pencil_variant(int Nq) {
@@ -211,7 +228,8 @@ int configure_launch(dgk_solver *s) {
                           (s->Nq == 2 || s->Nq == 4 || s->Nq == 8);
   if (use_pencil)
     s->sk = pick_pencil(s->Nq, s->order, s->fma, (s->flags & DGK_FLAG_FACTORED_LIFT) != 0,
-                        s->variant >= 0 ? s->variant : default_pencil_variant(s->Nq));
+                        (s->variant >= 0 ? s->variant : default_pencil_variant(s->Nq)) &
+                            (kPencilTemplateVariants - 1));
   else
     s->sk = pick_stage(s->Nq, s->order, s->fma, s->structured);
   if (!s->sk.kernel) return fail(DGK_ERR_ORDER, "Nq %d not supported", s->Nq);
@@ -223,7 +241,14 @@ int configure_launch(dgk_solver *s) {
   const long long ngroups = (s->Ne + s->sk.epb - 1) / s->sk.epb;
   if (s->sk.pencil) {
     // one workgroup per element group, remapped per XCD inside the kernel
-    s->sk.grid = (ngroups + kNumXcd - 1) / kNumXcd * kNumXcd;
+    if (tiled_order(s)) {
+      int tx, ty;
+      long long tseq;
+      tile_shape(s, &tx, &ty, &tseq);
+      s->sk.grid = (tseq + s->sk.epb - 1) / s->sk.epb * kNumXcd;
+    } else {
+      s->sk.grid = (ngroups + kNumXcd - 1) / kNumXcd * kNumXcd;
+    }
     return DGK_OK;
   }
   int dev = 0, ncu = 0, per_cu = 0;
@@ -264,6 +289,12 @@ int launch_halo(dgk_solver *s, double *f) {
 
 StageParams make_params(dgk_solver *s, int mode, double a, double b, double dt) {
   StageParams P{};
+  if (tiled_order(s)) {
+    P.tnx = s->NeX;
+    P.tny = s->NeY;
+    P.tnz = s->NeZ;
+    tile_shape(s, &P.ttx, &P.tty, &P.tseq);
+  }
   P.qin = s->q[s->cur];
   P.qout = s->q[s->cur ^ 1];
   P.q0 = s->q0;
@@ -356,6 +387,31 @@ bool compress_tables(int Nq, long long Ne, const int32_t *VMapM, const int32_t *
     }
   }
   return true;
+}
+
+// When a compressed neighbour table is exactly the periodic structured mesh
+// of the reference's numbering (mod_mesh.f90:486-493), records its extents
+// so that the pencil kernel may walk it in tiles.  The -x, -y and -z
+// neighbours of element 1 are (NeX,1,1), (1,NeY,1) and (1,1,NeZ).
+void infer_structured_extents(dgk_solver *s, const std::vector<int32_t> &nbr) {
+  const long long nx = (long long)nbr[3] + 1;
+  if (nx < 1 || s->Ne % nx) return;
+  const long long ny = nbr[0] / nx + 1;
+  if (ny < 1 || (s->Ne / nx) % ny) return;
+  const long long nz = s->Ne / (nx * ny);
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        const long long e = i + nx * (j + ny * k);
+        for (int f = 0; f < 6; ++f) {
+          bool wrapped;
+          if (nbr[e * 6 + f] != neighbour_elem(f, i, j, k, (int)nx, (int)ny, (int)nz, &wrapped))
+            return;
+        }
+      }
+  s->NeX = (int)nx;
+  s->NeY = (int)ny;
+  s->NeZ = (int)nz;
 }
 
 int finish_create(dgk_solver *s, const double *D1D, const double *Lift) {
@@ -474,6 +530,7 @@ int dgk_create(dgk_solver **out, int Nq, int Ne, int NeA, int NfpTot, const doub
                   compress_tables(Nq, Ne, VMapM, VMapP, normal_fn, Escale, Fscale, halo_src_map,
                                   NhaloNode, nbr, s->fs, s->es);
   if (s->structured) {
+    infer_structured_extents(s, nbr);
     rc = upload(&s->nbr, nbr.data(), nbr.size(), s->stream);
   } else {
     if (!rc) rc = upload(&s->vmapM, VMapM, (size_t)NfNe, s->stream);
@@ -518,6 +575,9 @@ int dgk_create_structured(dgk_solver **out, int NeX, int NeY, int NeZ, int p, do
   s->fma = (flags & DGK_FLAG_FMA) != 0;
   s->flags = flags;
   s->structured = true;
+  s->NeX = NeX;
+  s->NeY = NeY;
+  s->NeZ = NeZ;
   if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
     delete s;
     return fail(DGK_ERR_HIP, "hipStreamCreate failed");
@@ -680,7 +740,8 @@ int dgk_query(dgk_solver *s, int what, int64_t *value) {
   case 5: *value = s->sk.nt; break;
   case 6: *value = s->sk.pencil ? 1 : 0; break;
   case 7: *value = (int64_t)s->sk.lds; break;
-  case 8: *value = s->sk.variant; break;
+  case 8: *value = s->variant >= 0 ? s->variant : default_pencil_variant(s->Nq); break;
+  case 9: *value = tiled_order(s) ? 1 : 0; break;
   default: return fail(DGK_ERR_ARG, "unknown query %d", what);
   }
   return DGK_OK;
@@ -688,7 +749,7 @@ int dgk_query(dgk_solver *s, int what, int64_t *value) {
 
 int dgk_num_variants(dgk_solver *s, int *n) {
   if (!s || !n) return fail(DGK_ERR_ARG, "null argument");
-  *n = s->sk.pencil ? kPencilVariants : 1;
+  *n = s->sk.pencil ? kPencilVariants : 1;  // template variants x {tiled, linear order}
   return DGK_OK;
 }
 

@@ -42,7 +42,36 @@ struct StageParams {
   long long Ne;
   double a, b, dt;
   int mode;
+  // Tiled element order (pencil kernel) for a periodic NeX x NeY x NeZ mesh
+  // in the reference's numbering: tnx = 0 selects the plain linear order.
+  int tnx, tny, tnz;  // mesh extents
+  int ttx, tty;       // tile extents in x and y
+  long long tseq;     // sequence positions per XCD
 };
+
+// Element at sequence position s of XCD x in the tiled order, or -1.
+// XCD x owns the z-slab k in [x*NeZ/8, (x+1)*NeZ/8) and walks it tile by
+// tile (ttx x tty elements in x,y), each tile plane by plane, so that the
+// x-, y- and z-neighbours of an element are processed within one tile plane
+// (ttx*tty elements) of it by the same XCD, whose L2 still holds their data.
+__device__ __forceinline__ long long tiled_element(const StageParams &P, int x, long long s) {
+  const int k0 = (int)((long long)x * P.tnz / kNumXcd);
+  const int k1 = (int)((long long)(x + 1) * P.tnz / kNumXcd);
+  const int nk = k1 - k0;
+  const int plane = P.ttx * P.tty;
+  const long long per_tile = (long long)plane * nk;
+  if (nk <= 0 || s >= P.tseq) return -1;
+  const long long t = s / per_tile;
+  const long long r = s - t * per_tile;
+  const int k = k0 + (int)(r / plane);
+  const int rr = (int)(r - (long long)(k - k0) * plane);
+  const int jl = rr / P.ttx, il = rr - jl * P.ttx;
+  const int ntx = (P.tnx + P.ttx - 1) / P.ttx;
+  const long long tj = t / ntx, ti = t - tj * ntx;
+  const long long i = ti * P.ttx + il, j = tj * P.tty + jl;
+  if (i >= P.tnx || j >= P.tny) return -1;
+  return i + (long long)P.tnx * (j + (long long)P.tny * k);
+}
 
 // ---------------------------------------------------------------------------
 // Arithmetic helpers.  madd(x,y,s) = s + x*y; FMA variants fuse it.

@@ -105,9 +105,14 @@ __device__ __forceinline__ void sweep(const double *Drow, const double (&f)[NQ],
 //                     that their latency overlaps those phases (costs VGPRs)
 //   kPencilNtStore    write q, q0 and dqdt with non-temporal stores (no reuse
 //                     within a launch)
+//   kPencilLinearOrder  linear element order instead of the tiled order on
+//                       structured meshes (a launch parameter, not a
+//                       template argument)
 constexpr int kPencilLiftEarly = 1;
 constexpr int kPencilNtStore = 2;
-constexpr int kPencilVariants = 4;
+constexpr int kPencilLinearOrder = 4;
+constexpr int kPencilTemplateVariants = 4;
+constexpr int kPencilVariants = 8;
 
 template <bool NT>
 __device__ __forceinline__ void st(double *p, double v) {
@@ -134,15 +139,30 @@ __global__ void __launch_bounds__(64)
   double *Bv = Bu + PAD;  // v, then D_y
   double *Bf = Bv + PAD;  // fluxes of faces -y, +y, -x, +x
 
-  // XCD-aware element order: workgroup b runs on XCD b % 8; XCD x walks the
-  // x-th eighth of the element groups in order.
-  const long long ngroups = (P.Ne + EPW - 1) / EPW;
-  const long long per = (ngroups + kNumXcd - 1) / kNumXcd;
-  const long long g = (long long)(blockIdx.x % kNumXcd) * per + blockIdx.x / kNumXcd;
-  if (g >= ngroups) return;  // uniform over the workgroup
-  const long long e = g * EPW + el;
-  const bool valid = e < P.Ne;
-  const long long eb = valid ? e : g * EPW;  // a valid element for address math
+  // XCD-aware element order: workgroup b runs on XCD b % 8.  Linear order:
+  // XCD x walks the x-th eighth of the element groups in order.  Tiled order
+  // (structured meshes): see tiled_element.
+  const int xcd = blockIdx.x % kNumXcd;
+  const long long g = blockIdx.x / kNumXcd;
+  long long e, e_any;
+  if (P.tnx) {
+    if (g * EPW >= P.tseq) return;  // uniform over the workgroup
+    e = tiled_element(P, xcd, g * EPW + el);
+    // any valid element of this workgroup, for the address math of idle lanes
+    e_any = e;
+    for (int t = 0; t < EPW && e_any < 0; ++t) e_any = tiled_element(P, xcd, g * EPW + t);
+    if (e_any < 0) return;  // uniform: no lane of this workgroup has an element
+  } else {
+    const long long ngroups = (P.Ne + EPW - 1) / EPW;
+    const long long per = (ngroups + kNumXcd - 1) / kNumXcd;
+    const long long gl = (long long)xcd * per + g;
+    if (gl >= ngroups) return;  // uniform over the workgroup
+    e = gl * EPW + el;
+    if (e >= P.Ne) e = -1;
+    e_any = gl * EPW;
+  }
+  const bool valid = e >= 0;
+  const long long eb = valid ? e : e_any;  // a valid element for address math
 
   // ---- loads: own z-line (i,j) = (la,lb) and six neighbour face traces ----
   const long long base = eb * NP + pl;

@@ -245,10 +245,12 @@ def test_stretched_mesh_keeps_tables(gpu):
 
 
 @pytest.mark.parametrize("p", [1, 3, 7])
-def test_every_tuning_variant_is_bit_exact(gpu, golden, p):
-    """All code shapes the autotuner may pick give the reference's bits."""
-    case = [c for c in CS.RUN_CASES if c[1] == p] or [((4, 3, 2), p, 3, (0.7, 0.2, -0.5), "rand")]
-    ne, p, nstep, vel, kind = case[0]
+@pytest.mark.parametrize("ne", [(3, 2, 9), (4, 5, 3)])
+def test_every_tuning_variant_is_bit_exact(gpu, p, ne):
+    """All code shapes the autotuner may pick (tiled and linear element
+    orders included: NeZ = 9 gives uneven z-slabs per XCD) give the
+    reference's bits."""
+    nstep, vel, kind = 3, (0.7, 0.2, -0.5), "rand"
     x1D, D, L = dgk.read_operator(p)
     m = dgk.mesh_setup(*ne, p, x1D=x1D, tables=False)
     q, u, v, w, dt = CS.run_inputs(ne, p, vel, kind, m["pos_en"])
@@ -257,9 +259,10 @@ def test_every_tuning_variant_is_bit_exact(gpu, golden, p):
     om.run(*F, dt, nstep, O.OPT1)
     s = dgk.Solver.structured(*ne, p, D, L)
     for var in range(s.num_variants):
-        say(f"[variant] p={p} variant={var}")
+        say(f"[variant] {ne} p={p} variant={var}")
         s.set_variant(var)
         assert s.variant == var
+        assert s.query(9) == (1 if ne[2] >= 8 and not var & 4 else 0)
         s.set_fields(q, u, v, w)
         s.run(dt, nstep)
         assert np.array_equal(s.get_field(), F[0][:, :om.Ne]), var
