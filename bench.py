@@ -47,6 +47,9 @@ VARIANTS = {
     "fma": dgk.FLAG_FMA,                               # fused multiply-adds
     "fma_factored": dgk.FLAG_FMA | dgk.FLAG_FACTORED_LIFT,
     "exact_node": dgk.FLAG_NODE_KERNEL,                # thread-per-node kernel
+    # uniform velocity compressed to 3 scalars: 24 B per DOF-update
+    "factored_constvel": dgk.FLAG_FACTORED_LIFT | dgk.FLAG_CONST_VELOCITY,
+    "exact_constvel": dgk.FLAG_CONST_VELOCITY,
     "fma_node": dgk.FLAG_FMA | dgk.FLAG_NODE_KERNEL,
 }
 

@@ -52,6 +52,10 @@ extern "C" {
 #define DGK_FLAG_FACTORED_LIFT 0x8 /* lift with the 1D vectors
                                       Lift(:,1,1,4) and Lift(:,1,1,2) (exact
                                       to ~1e-14 relative, not bit-identical) */
+#define DGK_FLAG_CONST_VELOCITY 0x10 /* u,v,w uniform (checked by
+                                      dgk_set_fields): only q and q0 stream,
+                                      24 B per DOF-update instead of 48; same
+                                      results (pencil kernel only)          */
 
 /* Fields addressable through dgk_get_field / dgk_field_ptr. */
 #define DGK_FIELD_Q 0

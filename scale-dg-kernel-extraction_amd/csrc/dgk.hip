@@ -44,7 +44,6 @@ struct StageKernel {
   size_t lds = 0;
   int epb = 1;       // elements per workgroup (node kernel) or wavefront (pencil)
   bool pencil = false;
-  int variant = 0;   // tuning variant (pencil kernel)
   long long grid = 0;
 };
 
@@ -66,51 +65,56 @@ StageKernel make_stage() {
   return k;
 }
 
-template <int NQ, int ORDER, bool FMA, bool FACT, int VAR>
+template <int NQ, int ORDER, bool FMA, bool FACT, bool NT, bool CV>
 void launch_pencil(const StageKernel &k, const StageParams &P, const HostOps &H,
                    const double *Lift, hipStream_t st) {
-  hipLaunchKernelGGL((pencil_kernel<NQ, ORDER, FMA, FACT, VAR>), dim3((unsigned)k.grid),
+  (void)Lift;
+  hipLaunchKernelGGL((pencil_kernel<NQ, ORDER, FMA, FACT, NT, CV>), dim3((unsigned)k.grid),
                      dim3(64), k.lds, st, P, H.Drow, H.LiftP, H.lfac);
 }
 
-template <int NQ, int ORDER, bool FMA, bool FACT, int VAR>
+template <int NQ, int ORDER, bool FMA, bool FACT, bool NT, bool CV>
 StageKernel make_pencil() {
   StageKernel k;
-  k.kernel = (const void *)pencil_kernel<NQ, ORDER, FMA, FACT, VAR>;
-  k.launch = launch_pencil<NQ, ORDER, FMA, FACT, VAR>;
+  k.kernel = (const void *)pencil_kernel<NQ, ORDER, FMA, FACT, NT, CV>;
+  k.launch = launch_pencil<NQ, ORDER, FMA, FACT, NT, CV>;
   k.nt = 64;
   k.lds = PencilShape<NQ>::LDS;
   k.epb = PencilShape<NQ>::EPW;
   k.pencil = true;
-  k.variant = VAR;
   return k;
 }
 
+// Runtime choices to template arguments, one level at a time.
+template <int NQ, int ORDER, bool FMA, bool FACT, bool NT>
+StageKernel pick_pencil_cv(bool cv) {
+  return cv ? make_pencil<NQ, ORDER, FMA, FACT, NT, true>()
+            : make_pencil<NQ, ORDER, FMA, FACT, NT, false>();
+}
 template <int NQ, int ORDER, bool FMA, bool FACT>
-StageKernel pick_pencil_var(int var) {
-  switch (var) {
-  case 0: return make_pencil<NQ, ORDER, FMA, FACT, 0>();
-  case 1: return make_pencil<NQ, ORDER, FMA, FACT, 1>();
-  case 2: return make_pencil<NQ, ORDER, FMA, FACT, 2>();
-  default: return make_pencil<NQ, ORDER, FMA, FACT, 3>();
-  }
+StageKernel pick_pencil_nt(bool nt, bool cv) {
+  return nt ? pick_pencil_cv<NQ, ORDER, FMA, FACT, true>(cv)
+            : pick_pencil_cv<NQ, ORDER, FMA, FACT, false>(cv);
 }
-
+template <int NQ, int ORDER, bool FMA>
+StageKernel pick_pencil_fact(bool fact, bool nt, bool cv) {
+  return fact ? pick_pencil_nt<NQ, ORDER, FMA, true>(nt, cv)
+              : pick_pencil_nt<NQ, ORDER, FMA, false>(nt, cv);
+}
 template <int NQ>
-StageKernel pick_pencil_nq(int order, bool fma, bool fact, int var) {
-  if (order == kGeneral) {
-    if (fma) return fact ? pick_pencil_var<NQ, kGeneral, true, true>(var) : pick_pencil_var<NQ, kGeneral, true, false>(var);
-    return fact ? pick_pencil_var<NQ, kGeneral, false, true>(var) : pick_pencil_var<NQ, kGeneral, false, false>(var);
-  }
-  if (fma) return fact ? pick_pencil_var<NQ, kOpt1, true, true>(var) : pick_pencil_var<NQ, kOpt1, true, false>(var);
-  return fact ? pick_pencil_var<NQ, kOpt1, false, true>(var) : pick_pencil_var<NQ, kOpt1, false, false>(var);
+StageKernel pick_pencil_nq(int order, bool fma, bool fact, bool nt, bool cv) {
+  if (order == kGeneral)
+    return fma ? pick_pencil_fact<NQ, kGeneral, true>(fact, nt, cv)
+               : pick_pencil_fact<NQ, kGeneral, false>(fact, nt, cv);
+  return fma ? pick_pencil_fact<NQ, kOpt1, true>(fact, nt, cv)
+             : pick_pencil_fact<NQ, kOpt1, false>(fact, nt, cv);
 }
 
-StageKernel pick_pencil(int Nq, int order, bool fma, bool fact, int var) {
+StageKernel pick_pencil(int Nq, int order, bool fma, bool fact, bool nt, bool cv) {
   switch (Nq) {
-  case 2: return pick_pencil_nq<2>(order, fma, fact, var);
-  case 4: return pick_pencil_nq<4>(order, fma, fact, var);
-  case 8: return pick_pencil_nq<8>(order, fma, fact, var);
+  case 2: return pick_pencil_nq<2>(order, fma, fact, nt, cv);
+  case 4: return pick_pencil_nq<4>(order, fma, fact, nt, cv);
+  case 8: return pick_pencil_nq<8>(order, fma, fact, nt, cv);
   default: return StageKernel{};
   }
 }
@@ -181,6 +185,7 @@ struct dgk_solver {
   int flags = 0;
   int variant = -1;  // pencil tuning variant; -1 = built-in default
   int NeX = 0, NeY = 0, NeZ = 0;  // structured extents in the reference numbering (0: unknown)
+  double cvel[3] = {0, 0, 0};     // uniform velocity (DGK_FLAG_CONST_VELOCITY)
   HostOps ops;
   double *q[2] = {nullptr, nullptr};
   double *q0 = nullptr, *u = nullptr, *v = nullptr, *w = nullptr, *dqdt = nullptr;
@@ -226,10 +231,13 @@ int default_pencil_variant(int Nq) {
 int configure_launch(dgk_solver *s) {
   const bool use_pencil = s->structured && !(s->flags & DGK_FLAG_NODE_KERNEL) &&
                           (s->Nq == 2 || s->Nq == 4 || s->Nq == 8);
+  const int var = s->variant >= 0 ? s->variant : default_pencil_variant(s->Nq);
+  if ((s->flags & DGK_FLAG_CONST_VELOCITY) && !use_pencil)
+    return fail(DGK_ERR_ARG, "DGK_FLAG_CONST_VELOCITY needs the pencil kernel (Nq 2, 4 or 8, "
+                             "compressed connectivity)");
   if (use_pencil)
     s->sk = pick_pencil(s->Nq, s->order, s->fma, (s->flags & DGK_FLAG_FACTORED_LIFT) != 0,
-                        (s->variant >= 0 ? s->variant : default_pencil_variant(s->Nq)) &
-                            (kPencilTemplateVariants - 1));
+                        (var & kPencilNtStore) != 0, (s->flags & DGK_FLAG_CONST_VELOCITY) != 0);
   else
     s->sk = pick_stage(s->Nq, s->order, s->fma, s->structured);
   if (!s->sk.kernel) return fail(DGK_ERR_ORDER, "Nq %d not supported", s->Nq);
@@ -312,6 +320,7 @@ StageParams make_params(dgk_solver *s, int mode, double a, double b, double dt) 
   P.escale = s->escale;
   for (int f = 0; f < 6; ++f) P.fs[f] = s->fs[f];
   for (int d = 0; d < 3; ++d) P.es[d] = s->es[d];
+  for (int d = 0; d < 3; ++d) P.cvel[d] = s->cvel[d];
   P.Ne = s->Ne;
   P.a = a;
   P.b = b;
@@ -613,6 +622,20 @@ int dgk_set_fields(dgk_solver *s, const double *q, const double *u, const double
                    const double *w) {
   if (!s || !q || !u || !v || !w) return fail(DGK_ERR_ARG, "null argument");
   const size_t nb = (size_t)s->Np * s->NeA * sizeof(double);
+  if (s->flags & DGK_FLAG_CONST_VELOCITY) {
+    // the compressed form streams no velocity: every owned node must carry
+    // the same (u,v,w)
+    const double *vel[3] = {u, v, w};
+    const size_t owned = (size_t)s->Np * s->Ne;
+    for (int d = 0; d < 3; ++d) {
+      for (size_t n = 1; n < owned; ++n)
+        if (vel[d][n] != vel[d][0])
+          return fail(DGK_ERR_ARG, "DGK_FLAG_CONST_VELOCITY: velocity component %d is not "
+                                   "uniform (node %zu)", d + 1, n + 1);
+      s->cvel[d] = vel[d][0];
+    }
+    drop_graphs(s);  // the velocity is a launch argument
+  }
   s->cur = 0;
   HIPOK(hipMemcpyAsync(s->q[0], q, nb, hipMemcpyHostToDevice, s->stream));
   HIPOK(hipMemcpyAsync(s->u, u, nb, hipMemcpyHostToDevice, s->stream));

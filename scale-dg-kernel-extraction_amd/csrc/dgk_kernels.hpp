@@ -39,6 +39,7 @@ struct StageParams {
   const double *escale;          // (Np,Ne,3)
   double fs[6];                  // compressed: Fscale per face
   double es[3];                  // compressed: Escale per axis
+  double cvel[3];                // constant velocity (pencil kernel, CV)
   long long Ne;
   double a, b, dt;
   int mode;

@@ -101,18 +101,14 @@ __device__ __forceinline__ void sweep(const double *Drow, const double (&f)[NQ],
 }
 
 // Tuning variants of the pencil kernel (bit set):
-//   kPencilLiftEarly  issue the dense Lift loads before the y and x phases so
-//                     that their latency overlaps those phases (costs VGPRs)
-//   kPencilNtStore    write q, q0 and dqdt with non-temporal stores (no reuse
-//                     within a launch)
+//   kPencilNtStore      write q, q0 and dqdt with non-temporal stores (no
+//                       reuse within a launch)
 //   kPencilLinearOrder  linear element order instead of the tiled order on
 //                       structured meshes (a launch parameter, not a
 //                       template argument)
-constexpr int kPencilLiftEarly = 1;
-constexpr int kPencilNtStore = 2;
-constexpr int kPencilLinearOrder = 4;
-constexpr int kPencilTemplateVariants = 4;
-constexpr int kPencilVariants = 8;
+constexpr int kPencilNtStore = 1;
+constexpr int kPencilLinearOrder = 2;
+constexpr int kPencilVariants = 4;
 
 template <bool NT>
 __device__ __forceinline__ void st(double *p, double v) {
@@ -120,13 +116,14 @@ __device__ __forceinline__ void st(double *p, double v) {
   else *p = v;
 }
 
-template <int NQ, int ORDER, bool FMA, bool FACT_LIFT, int VAR>
+// CV: constant velocity (P.cvel): only q and q0 stream through HBM; the
+// products q*u etc. and the face velocities are those of uniform u,v,w
+// fields, so the results are the same bits.
+template <int NQ, int ORDER, bool FMA, bool FACT_LIFT, bool NT, bool CV>
 __global__ void __launch_bounds__(64)
     pencil_kernel(const StageParams P, const double *__restrict__ Drow,
                   const double *__restrict__ Lift, const double *__restrict__ lfac) {
-  // Lift: the pair-packed copy described at load_lift below
-  constexpr bool LIFT_EARLY = (VAR & kPencilLiftEarly) && !FACT_LIFT;
-  constexpr bool NT = (VAR & kPencilNtStore) != 0;
+  // Lift: the pair-packed copy described at the lift loads below
   using S = PencilShape<NQ>;
   constexpr int NP = S::NP, NFP = S::NFP, EPW = S::EPW, PAD = S::PAD, FPAD = S::FPAD;
   extern __shared__ double smem[];
@@ -171,9 +168,15 @@ __global__ void __launch_bounds__(64)
 #pragma unroll
   for (int k = 0; k < NQ; ++k) {
     q[k] = P.qin[base + NFP * k];
-    u[k] = P.u[base + NFP * k];
-    v[k] = P.v[base + NFP * k];
-    w[k] = P.w[base + NFP * k];
+    if constexpr (CV) {
+      u[k] = P.cvel[0];
+      v[k] = P.cvel[1];
+      w[k] = P.cvel[2];
+    } else {
+      u[k] = P.u[base + NFP * k];
+      v[k] = P.v[base + NFP * k];
+      w[k] = P.w[base + NFP * k];
+    }
     q0[k] = need_q0 ? P.q0[base + NFP * k] : 0.0;
   }
   const int32_t *nb = P.nbr + eb * 6;
@@ -187,9 +190,12 @@ __global__ void __launch_bounds__(64)
   // x faces at (j,k) = (la,lb): -x (face 3) sees i = NQ-1, +x (face 1) i = 0
   const long long a3 = (long long)nb[3] * NP + (NQ - 1) + NQ * pl;
   const long long a1 = (long long)nb[1] * NP + NQ * pl;
-  const double qP4 = P.qin[a4], wP4 = P.w[a4], qP5 = P.qin[a5], wP5 = P.w[a5];
-  const double qP0 = P.qin[a0], vP0 = P.v[a0], qP2 = P.qin[a2], vP2 = P.v[a2];
-  const double qP3 = P.qin[a3], uP3 = P.u[a3], qP1 = P.qin[a1], uP1 = P.u[a1];
+  const double qP4 = P.qin[a4], qP5 = P.qin[a5];
+  const double qP0 = P.qin[a0], qP2 = P.qin[a2];
+  const double qP3 = P.qin[a3], qP1 = P.qin[a1];
+  const double wP4 = CV ? P.cvel[2] : P.w[a4], wP5 = CV ? P.cvel[2] : P.w[a5];
+  const double vP0 = CV ? P.cvel[1] : P.v[a0], vP2 = CV ? P.cvel[1] : P.v[a2];
+  const double uP3 = CV ? P.cvel[0] : P.u[a3], uP1 = CV ? P.cvel[0] : P.u[a1];
 
   // ---- z phase (registers) ---------------------------------------------
   double dz[NQ];
@@ -205,21 +211,9 @@ __global__ void __launch_bounds__(64)
   for (int k = 0; k < NQ; ++k) {
     const int s = pidx<NQ>(pl + NFP * k);
     Bq[s] = q[k];
-    Bu[s] = u[k];
-    Bv[s] = v[k];
-  }
-  // Dense lift coefficients of this lane's z-line, Lift(i,j,k,f) for the
-  // 6*NQ pairs (k,f), packed by the host as 16-byte pairs that the lanes of a
-  // wavefront read contiguously: Lift[(m*NFP + line)*2 + {0,1}] holds
-  // values 2m and 2m+1 of the line, value v = 6k + f.
-  const double2 *LP = reinterpret_cast<const double2 *>(Lift);
-  double Le[LIFT_EARLY ? 6 * NQ : 1];
-  if constexpr (LIFT_EARLY) {
-#pragma unroll
-    for (int m = 0; m < 3 * NQ; ++m) {
-      const double2 pr = LP[m * NFP + pl];
-      Le[2 * m] = pr.x;
-      Le[2 * m + 1] = pr.y;
+    if constexpr (!CV) {
+      Bu[s] = u[k];
+      Bv[s] = v[k];
     }
   }
   __syncthreads();
@@ -231,7 +225,7 @@ __global__ void __launch_bounds__(64)
     for (int l = 0; l < NQ; ++l) {
       const int s = pidx<NQ>(la + NQ * l + NFP * lb);
       qy[l] = Bq[s];
-      vy[l] = Bv[s];
+      vy[l] = CV ? P.cvel[1] : Bv[s];
       fy[l] = qy[l] * vy[l];
     }
     sweep<NQ, ORDER, FMA, false>(Drow, fy, dy);
@@ -248,7 +242,7 @@ __global__ void __launch_bounds__(64)
     for (int l = 0; l < NQ; ++l) {
       const int s = pidx<NQ>(l + NQ * pl);
       qx[l] = Bq[s];
-      ux[l] = Bu[s];
+      ux[l] = CV ? P.cvel[0] : Bu[s];
       fx[l] = qx[l] * ux[l];
     }
     sweep<NQ, ORDER, FMA, true>(Drow, fx, dx);
@@ -281,10 +275,12 @@ __global__ void __launch_bounds__(64)
       L[3] = lfac[i];
       L[4] = lfac[k];
       L[5] = lfac[NQ + k];
-    } else if constexpr (LIFT_EARLY) {
-#pragma unroll
-      for (int f = 0; f < 6; ++f) L[f] = Le[6 * k + f];  // Lift(i,j,k,f)
     } else {
+      // Dense lift coefficients Lift(i,j,k,f) of this lane's z-line, packed
+      // by the host as 16-byte pairs that the lanes of a wavefront read
+      // contiguously: Lift[(m*NFP + line)*2 + {0,1}] holds values 2m, 2m+1
+      // of the line, value v = 6k + f.
+      const double2 *LP = reinterpret_cast<const double2 *>(Lift);
 #pragma unroll
       for (int m = 0; m < 3; ++m) {  // values 6k..6k+5 = pairs 3k..3k+2
         const double2 pr = LP[(3 * k + m) * NFP + pl];

@@ -262,10 +262,40 @@ def test_every_tuning_variant_is_bit_exact(gpu, p, ne):
         say(f"[variant] {ne} p={p} variant={var}")
         s.set_variant(var)
         assert s.variant == var
-        assert s.query(9) == (1 if ne[2] >= 8 and not var & 4 else 0)
+        assert s.query(9) == (1 if ne[2] >= 8 and not var & 2 else 0)
         s.set_fields(q, u, v, w)
         s.run(dt, nstep)
         assert np.array_equal(s.get_field(), F[0][:, :om.Ne]), var
+    s.close()
+
+
+@pytest.mark.parametrize("p", [1, 3, 7])
+@pytest.mark.parametrize("factored", [False, True])
+def test_constant_velocity_form(gpu, p, factored):
+    """DGK_FLAG_CONST_VELOCITY streams no velocity and gives the same bits as
+    the per-node velocity path; a non-uniform velocity is refused."""
+    ne, nstep, vel = (3, 4, 8), 4, (0.45, -0.8, 0.3)
+    x1D, D, L = dgk.read_operator(p)
+    m = dgk.mesh_setup(*ne, p, x1D=x1D, tables=False)
+    q = CS.bump_field(m["pos_en"], 123, noise=0.01)
+    dt = ic.cfl_dt(p, vel, ne)
+    uvw = [np.full_like(q, c) for c in vel]
+    base = dgk.FLAG_FACTORED_LIFT if factored else 0
+    outs = []
+    for flags in (base, base | dgk.FLAG_CONST_VELOCITY):
+        say(f"[const velocity] p={p} flags={flags}")
+        s = dgk.Solver.structured(*ne, p, D, L, flags=flags)
+        s.set_fields(q, *uvw)
+        s.run(dt, nstep)
+        outs.append(s.get_field())
+        s.close()
+    assert np.array_equal(outs[0], outs[1])
+    s = dgk.Solver.structured(*ne, p, D, L, flags=dgk.FLAG_CONST_VELOCITY)
+    bad = uvw[1].copy(order="F")
+    bad[5, 2] += 1e-3
+    with pytest.raises(dgk.DGKError) as e:
+        s.set_fields(q, uvw[0], bad, uvw[2])
+    assert e.value.code == dgk.ERR_ARG
     s.close()
 
 
