@@ -187,6 +187,44 @@ int dgk_divlike_dirXYZ_batched(int Nq, int nelem, int type_id,
 int dgk_lift_hexahedral_batched(int Nq, int nelem, const double *Lift,
                                 const double *in, double *out, void *stream);
 
+/* ---- batched ensemble ---------------------------------------------------- */
+/* n independent runs of the reference program (main.f90:51-80), each on its
+ * own periodic NeX x NeY x NeZ mesh of order p with its own velocity field,
+ * initial state and dt, advanced together: one persistent launch per RK
+ * stage over all instances, the three stages of a step captured in one
+ * hipGraph.  Every instance gives the same bits as a dgk_create_structured
+ * solver with the same flags (OPT1 summation order).  PolyOrder 1, 3, 5, 7
+ * or 11.  dom: n triples of domain lengths, or NULL for unit cubes.
+ * operator_dir: the directory of the reference's pN.dat files.  flags:
+ * DGK_FLAG_FMA, DGK_FLAG_FACTORED_LIFT, DGK_FLAG_CONST_VELOCITY. */
+typedef struct dgk_ensemble dgk_ensemble;
+int dgk_ensemble_create(dgk_ensemble **out, int n, const int32_t *p,
+                        const int32_t *NeX, const int32_t *NeY,
+                        const int32_t *NeZ, const double *dom,
+                        const char *operator_dir, int type_id, int flags);
+int dgk_ensemble_destroy(dgk_ensemble *s);
+/* instance i: q,u,v,w (Np,Ne) host arrays (main.f90:135-157) */
+int dgk_ensemble_set_fields(dgk_ensemble *s, int i, const double *q,
+                            const double *u, const double *v, const double *w);
+int dgk_ensemble_get_field(dgk_ensemble *s, int i, double *q);
+/* dt[n]: time step of every instance (main.f90:143) */
+int dgk_ensemble_set_dt(dgk_ensemble *s, const double *dt);
+/* one SSP-RK3 step of every instance (graph replay, asynchronous) */
+int dgk_ensemble_step(dgk_ensemble *s);
+/* nstep steps, then waits for the device */
+int dgk_ensemble_run(dgk_ensemble *s, int nstep);
+/* average device time of one stage launch over nstage launches (the state
+ * advances by those stages) */
+int dgk_ensemble_time_stages(dgk_ensemble *s, int nstage, double *ms_per_stage);
+/* Measures each order class alone (nrep stage launches; the state is left
+ * unchanged) and splits the persistent grid between the classes so that
+ * they finish together.  ms_per_class[number of classes] or NULL. */
+int dgk_ensemble_tune(dgk_ensemble *s, int nrep, double *ms_per_class);
+/* what: 0 total DOF, 1 work items per stage, 2 workgroups, 3 LDS bytes,
+ * 4 classes of equal order, 5 tuned (0/1), 16+c Nq of class c,
+ * 32+c workgroups of class c */
+int dgk_ensemble_query(dgk_ensemble *s, int what, int64_t *value);
+
 #ifdef __cplusplus
 }
 #endif

@@ -13,6 +13,67 @@
 
 using namespace dgk;
 
+namespace dgk {
+
+// update_halo (mod_mesh.f90:216-218): f(Np*Ne + ib) = f(halo_src_map(ib)).
+__global__ void halo_gather_kernel(double *f, const int32_t *src, int n, long long base) {
+  const long long ib = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (ib < n) f[base + ib] = f[src[ib] - 1];
+}
+
+// Two-pass (min,max) of the owned state: per-block partials, then one block.
+__global__ void __launch_bounds__(256)
+    minmax_partial_kernel(const double *q, long long n, double *part) {
+  __shared__ double smn[256], smx[256];
+  double mn = __builtin_inf(), mx = -__builtin_inf();
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (long long)gridDim.x * blockDim.x) {
+    const double x = q[t];
+    mn = x < mn ? x : mn;
+    mx = x > mx ? x : mx;
+  }
+  smn[threadIdx.x] = mn;
+  smx[threadIdx.x] = mx;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      smn[threadIdx.x] = fmin(smn[threadIdx.x], smn[threadIdx.x + s]);
+      smx[threadIdx.x] = fmax(smx[threadIdx.x], smx[threadIdx.x + s]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = smn[0];
+    part[2 * blockIdx.x + 1] = smx[0];
+  }
+}
+
+__global__ void __launch_bounds__(256)
+    minmax_final_kernel(const double *part, int nb, double *out) {
+  __shared__ double smn[256], smx[256];
+  double mn = __builtin_inf(), mx = -__builtin_inf();
+  for (int t = threadIdx.x; t < nb; t += blockDim.x) {
+    mn = fmin(mn, part[2 * t]);
+    mx = fmax(mx, part[2 * t + 1]);
+  }
+  smn[threadIdx.x] = mn;
+  smx[threadIdx.x] = mx;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      smn[threadIdx.x] = fmin(smn[threadIdx.x], smn[threadIdx.x + s]);
+      smx[threadIdx.x] = fmax(smx[threadIdx.x], smx[threadIdx.x + s]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = smn[0];
+    out[1] = smx[0];
+  }
+}
+
+}  // namespace dgk
+
 #define HIPOK(expr)                                                            \
   do {                                                                         \
     hipError_t e_ = (expr);                                                    \

@@ -1,0 +1,657 @@
+// Batched ensemble: many independent periodic meshes of uneven polynomial
+// order, size and velocity advanced together, one persistent launch per RK
+// stage and the three stages of a time step captured in one hipGraph.
+//
+// Each instance keeps the reference layout q(Np,Ne) inside pooled device
+// arrays (int64 offsets).  Work items (instance, first element) are grouped
+// into classes of equal order; the persistent grid is split between the
+// classes (in units of eight workgroups, one per XCD) and every workgroup
+// runs items of one class only, so that it is compiled and scheduled as one
+// straight code path.  A workgroup of 256 threads runs an item as
+//   Nq = 2, 4, 8   four wavefronts, each the pencil stage of its element(s);
+//   Nq = 6, 12     the cooperative thread-per-node stage of one element.
+// Within a class the items are split into eight contiguous ranges, one per
+// XCD, so that an instance's neighbouring elements stay in one L2.  The
+// split of the grid between the classes follows measured per-class stage
+// times (dgk_ensemble_tune) or, before tuning, the DOF count of each class.
+#include "dgk_internal.hpp"
+#include "dgk_kernels.hpp"
+#include "dgk_pencil.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <vector>
+
+using namespace dgk;
+
+#define HIPOK(expr)                                                            \
+  do {                                                                         \
+    hipError_t e_ = (expr);                                                    \
+    if (e_ != hipSuccess)                                                      \
+      return fail(DGK_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+namespace {
+
+constexpr int kEnsThreads = 256;
+
+struct EnsInst {
+  int nq, op;            // Nq and operator slot
+  long long ne;          // elements
+  long long off;         // field offset (doubles) in the pools
+  long long nbr_off;     // neighbour-table offset (int32)
+  double es[3], fs[6], cvel[3];
+  double dt;
+};
+
+struct EnsOps {          // per distinct Nq
+  const double *D, *Lift, *Drow, *LiftP, *lfac;
+};
+
+constexpr int kEnsMaxClass = 5;  // Nq 2, 4, 6, 8, 12
+
+struct EnsArgs {
+  const EnsInst *inst;
+  const EnsOps *ops;
+  const int2 *items;     // (instance, first element), grouped by class
+  const double *qin;
+  double *qout, *q0;
+  const double *u, *v, *w;
+  const int32_t *nbr;
+  double a, b;
+  int mode;
+  int ncls;
+  int cls_nq[kEnsMaxClass];
+  int cls_block[kEnsMaxClass + 1];  // first workgroup of each class (multiples of 8)
+  int cls_item[kEnsMaxClass + 1];   // first item of each class
+};
+
+bool ens_supported(int nq) { return nq == 2 || nq == 4 || nq == 6 || nq == 8 || nq == 12; }
+
+// elements per item for a given Nq
+int ens_item_elems(int nq) {
+  switch (nq) {
+  case 2: return 4 * PencilShape<2>::EPW;
+  case 4: return 4 * PencilShape<4>::EPW;
+  case 8: return 4 * PencilShape<8>::EPW;
+  case 6: return GroupShape<6, kEnsThreads>::EPB;
+  default: return GroupShape<12, kEnsThreads>::EPB;
+  }
+}
+
+size_t ens_item_lds(int nq) {
+  switch (nq) {
+  case 2: return 4 * PencilShape<2>::LDS;
+  case 4: return 4 * PencilShape<4>::LDS;
+  case 8: return 4 * PencilShape<8>::LDS;
+  case 6: return GroupShape<6, kEnsThreads>::LDS;
+  default: return GroupShape<12, kEnsThreads>::LDS;
+  }
+}
+
+// The operator tables of an item are read from memory the compiler cannot
+// prove constant, so their addresses arrive in VGPRs; the scalar row loads
+// of the pencil sweeps need them in SGPRs (they are uniform by construction).
+template <class T>
+__device__ __forceinline__ T *uniform_ptr(T *p) {
+  const unsigned long long v = (unsigned long long)p;
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+  return (T *)(((unsigned long long)hi << 32) | lo);
+}
+
+template <int NQ, int ORDER, bool FMA, bool FACT, bool CV>
+__device__ __forceinline__ void ens_pencil(const StageParams &P, const EnsOps &O, long long first,
+                                           double *smem) {
+  using S = PencilShape<NQ>;
+  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int el = lane / S::NFP, pl = lane - el * S::NFP;
+  long long e = first + (long long)wave * S::EPW + el;
+  if (e >= P.Ne) e = -1;
+  pencil_element<NQ, ORDER, FMA, FACT, true, CV>(P, uniform_ptr(O.Drow), O.LiftP, O.lfac,
+                                                  smem + (wave * S::EPW + el) * S::PER_ELEM, pl,
+                                                  e, first);
+}
+
+// The items [first, first+count) of one class on nb workgroups (lb = this
+// workgroup's index among them; lb % 8 is its XCD).
+template <int NQ, int ORDER, bool FMA, bool FACT, bool CV>
+__device__ __forceinline__ void ens_items(const EnsArgs &A, int first, int count, int lb, int nb,
+                                          double *smem) {
+  const int xcd = lb % kNumXcd, slot = lb / kNumXcd, nslot = nb / kNumXcd;
+  const int per = (count + kNumXcd - 1) / kNumXcd;
+  const int end = first + min(count, (xcd + 1) * per);
+  for (int it = first + xcd * per + slot; it < end; it += nslot) {
+    const int2 item = A.items[it];
+    const EnsInst &I = A.inst[item.x];
+    const EnsOps O = A.ops[I.op];
+    StageParams P{};
+    P.qin = A.qin + I.off;
+    P.qout = A.qout + I.off;
+    P.q0 = A.q0 + I.off;
+    P.u = A.u + I.off;
+    P.v = A.v + I.off;
+    P.w = A.w + I.off;
+    P.D = O.D;
+    P.Lift = O.Lift;
+    P.nbr = A.nbr + I.nbr_off;
+#pragma unroll
+    for (int f = 0; f < 6; ++f) P.fs[f] = I.fs[f];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      P.es[d] = I.es[d];
+      P.cvel[d] = I.cvel[d];
+    }
+    P.Ne = I.ne;
+    P.a = A.a;
+    P.b = A.b;
+    P.dt = I.dt;
+    P.mode = A.mode;
+    if constexpr (NQ == 2 || NQ == 4 || NQ == 8)
+      ens_pencil<NQ, ORDER, FMA, FACT, CV>(P, O, item.y, smem);
+    else
+      node_group<NQ, kEnsThreads, ORDER, FMA, FACT, CV>(
+          P, O.lfac, I.fs, I.cvel, item.y / GroupShape<NQ, kEnsThreads>::EPB, threadIdx.x, smem);
+    __syncthreads();  // LDS is reused by the next item
+  }
+}
+
+template <int ORDER, bool FMA, bool FACT, bool CV>
+__global__ void __launch_bounds__(kEnsThreads) ensemble_kernel(const EnsArgs A) {
+  extern __shared__ double smem[];
+  int c = 0;
+  while (c + 1 < A.ncls && (int)blockIdx.x >= A.cls_block[c + 1]) ++c;
+  const int b0 = A.cls_block[c], nb = A.cls_block[c + 1] - b0;
+  if ((int)blockIdx.x >= A.cls_block[A.ncls] || nb <= 0) return;  // uniform
+  const int lb = blockIdx.x - b0;
+  const int first = A.cls_item[c], count = A.cls_item[c + 1] - first;
+  switch (A.cls_nq[c]) {
+  case 2: ens_items<2, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
+  case 4: ens_items<4, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
+  case 6: ens_items<6, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
+  case 8: ens_items<8, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
+  default: ens_items<12, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
+  }
+}
+
+using EnsFn = void (*)(const EnsArgs);
+
+template <int ORDER, bool FMA, bool FACT>
+EnsFn pick_ens_cv(bool cv) {
+  return cv ? ensemble_kernel<ORDER, FMA, FACT, true> : ensemble_kernel<ORDER, FMA, FACT, false>;
+}
+template <int ORDER, bool FMA>
+EnsFn pick_ens_fact(bool fact, bool cv) {
+  return fact ? pick_ens_cv<ORDER, FMA, true>(cv) : pick_ens_cv<ORDER, FMA, false>(cv);
+}
+EnsFn pick_ens(bool fma, bool fact, bool cv) {
+  return fma ? pick_ens_fact<kOpt1, true>(fact, cv) : pick_ens_fact<kOpt1, false>(fact, cv);
+}
+
+const double kRkA[3] = {0.0, 0.75, 1.0 / 3.0};  // mod_common.f90:29
+const double kRkB[3] = {1.0, 0.25, 2.0 / 3.0};  // mod_common.f90:30
+
+}  // namespace
+
+struct dgk_ensemble {
+  int n = 0;
+  int flags = 0;
+  std::vector<EnsInst> inst;   // host copy
+  std::vector<int> np;         // nodes per element
+  long long total = 0;         // doubles per field pool
+  double *q[2] = {nullptr, nullptr};
+  double *q0 = nullptr, *u = nullptr, *v = nullptr, *w = nullptr;
+  int32_t *nbr = nullptr;
+  EnsInst *dinst = nullptr;
+  EnsOps *dops = nullptr;
+  int2 *ditems = nullptr;
+  int nitems = 0;
+  // classes of equal order and the split of the grid between them
+  int ncls = 0;
+  int cls_nq[kEnsMaxClass] = {};
+  int cls_item[kEnsMaxClass + 1] = {};
+  int cls_block[kEnsMaxClass + 1] = {};
+  double cls_weight[kEnsMaxClass] = {};  // DOF, or measured stage time once tuned
+  int slots = 0;                         // resident workgroups (multiple of 8)
+  bool tuned = false;
+  std::vector<double *> opbufs;
+  int cur = 0;
+  hipStream_t stream = nullptr;
+  EnsFn fn = nullptr;
+  size_t lds = 0;
+  int grid = 0;
+  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  bool fields_set = false;
+};
+
+namespace {
+
+void ens_drop_graphs(dgk_ensemble *s) {
+  for (auto &g : s->graph)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+}
+
+EnsArgs ens_args(const dgk_ensemble *s, int stage) {
+  EnsArgs A{};
+  A.inst = s->dinst;
+  A.ops = s->dops;
+  A.items = s->ditems;
+  A.qin = s->q[s->cur];
+  A.qout = s->q[s->cur ^ 1];
+  A.q0 = s->q0;
+  A.u = s->u;
+  A.v = s->v;
+  A.w = s->w;
+  A.nbr = s->nbr;
+  A.a = kRkA[stage];
+  A.b = kRkB[stage];
+  A.mode = stage == 0 ? kModeStage1 : kModeStageN;
+  A.ncls = s->ncls;
+  for (int c = 0; c < s->ncls; ++c) A.cls_nq[c] = s->cls_nq[c];
+  for (int c = 0; c <= s->ncls; ++c) {
+    A.cls_item[c] = s->cls_item[c];
+    A.cls_block[c] = s->cls_block[c];
+  }
+  return A;
+}
+
+int ens_launch_args(dgk_ensemble *s, const EnsArgs &A) {
+  const int grid = A.cls_block[A.ncls];
+  hipLaunchKernelGGL(s->fn, dim3(grid), dim3(kEnsThreads), s->lds, s->stream, A);
+  HIPOK(hipGetLastError());
+  return DGK_OK;
+}
+
+int ens_launch(dgk_ensemble *s, int stage) {
+  const int rc = ens_launch_args(s, ens_args(s, stage));
+  if (!rc) s->cur ^= 1;
+  return rc;
+}
+
+// Units of eight workgroups (one per XCD) class c can use: one per eight items.
+int ens_unit_cap(const dgk_ensemble *s, int c) {
+  return std::max(1, (s->cls_item[c + 1] - s->cls_item[c] + kNumXcd - 1) / kNumXcd);
+}
+
+// Splits the resident workgroups between the classes in proportion to
+// cls_weight (largest remaining weight per unit first), at least one unit
+// and at most ens_unit_cap units per class.
+void ens_partition(dgk_ensemble *s) {
+  const int units = std::max(s->slots / kNumXcd, s->ncls);
+  int u[kEnsMaxClass];
+  int used = 0;
+  for (int c = 0; c < s->ncls; ++c) {
+    u[c] = 1;
+    ++used;
+  }
+  while (used < units) {
+    int best = -1;
+    double bw = -1.0;
+    for (int c = 0; c < s->ncls; ++c) {
+      if (u[c] >= ens_unit_cap(s, c)) continue;
+      const double r = s->cls_weight[c] / u[c];
+      if (r > bw) {
+        bw = r;
+        best = c;
+      }
+    }
+    if (best < 0) break;
+    ++u[best];
+    ++used;
+  }
+  s->cls_block[0] = 0;
+  for (int c = 0; c < s->ncls; ++c) s->cls_block[c + 1] = s->cls_block[c] + kNumXcd * u[c];
+  s->grid = s->cls_block[s->ncls];
+}
+
+template <class T>
+int ens_alloc(T **p, size_t n) {
+  *p = nullptr;
+  if (!n) return DGK_OK;
+  if (hipMalloc((void **)p, n * sizeof(T)) != hipSuccess) {
+    *p = nullptr;
+    return fail(DGK_ERR_NOMEM, "hipMalloc of %zu bytes failed", n * sizeof(T));
+  }
+  return DGK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dgk_ensemble_destroy(dgk_ensemble *s) {
+  if (!s) return DGK_OK;
+  ens_drop_graphs(s);
+  void *bufs[] = {s->q[0], s->q[1], s->q0, s->u, s->v, s->w, s->nbr, s->dinst, s->dops, s->ditems};
+  for (void *b : bufs)
+    if (b) (void)hipFree(b);
+  for (double *b : s->opbufs)
+    if (b) (void)hipFree(b);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+  return DGK_OK;
+}
+
+int dgk_ensemble_create(dgk_ensemble **out, int n, const int32_t *p, const int32_t *nex,
+                        const int32_t *ney, const int32_t *nez, const double *dom,
+                        const char *operator_dir, int type_id, int flags) {
+  clear_error();
+  if (!out || n < 1 || !p || !nex || !ney || !nez || !operator_dir)
+    return fail(DGK_ERR_ARG, "bad argument");
+  *out = nullptr;
+  if (type_id != DGK_TYPE_OPT1 && type_id != DGK_TYPE_HIP)
+    return fail(DGK_ERR_OPTTYPE, "the ensemble evaluates in OPT1's summation order only");
+  if (flags & ~(DGK_FLAG_FMA | DGK_FLAG_FACTORED_LIFT | DGK_FLAG_CONST_VELOCITY))
+    return fail(DGK_ERR_ARG, "flags 0x%x not supported by the ensemble", flags);
+  for (int i = 0; i < n; ++i) {
+    int32_t sz[7];
+    int rc = dgk_mesh_sizes(nex[i], ney[i], nez[i], p[i], sz);
+    if (rc) return rc;
+    if (!ens_supported(p[i] + 1))
+      return fail(DGK_ERR_ORDER, "instance %d: the ensemble runs PolyOrder 1, 3, 5, 7 or 11 "
+                                 "(got %d)", i, p[i]);
+  }
+  dgk_ensemble *s = new dgk_ensemble;
+  s->n = n;
+  s->flags = flags;
+  s->inst.resize(n);
+  s->np.resize(n);
+  int rc = DGK_OK;
+  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete s;
+    return fail(DGK_ERR_HIP, "hipStreamCreate failed");
+  }
+  // operators, one slot per distinct order
+  std::map<int, int> slot;
+  std::vector<EnsOps> hops;
+  for (int i = 0; i < n && !rc; ++i) {
+    if (slot.count(p[i])) continue;
+    const int Nq = p[i] + 1, Np = Nq * Nq * Nq, Nfp = Nq * Nq;
+    std::vector<double> x1D(Nq), D(Nq * Nq), L(6 * (size_t)Np);
+    if ((rc = dgk_read_operator(operator_dir, p[i], x1D.data(), D.data(), L.data()))) break;
+    std::vector<double> Drow(Nq * Nq), LiftP(L.size()), lf(2 * Nq);
+    for (int k = 0; k < Nq; ++k)
+      for (int l = 0; l < Nq; ++l) Drow[k * Nq + l] = D[k + l * Nq];
+    for (int pl = 0; pl < Nfp; ++pl)
+      for (int v = 0; v < 6 * Nq; ++v) {
+        const int k = v / 6, f = v % 6, m = v / 2;
+        LiftP[((size_t)m * Nfp + pl) * 2 + (v & 1)] = L[pl + (size_t)Nfp * k + (size_t)Np * f];
+      }
+    for (int t = 0; t < Nq; ++t) {
+      lf[t] = L[t + (size_t)Np * 3];
+      lf[Nq + t] = L[t + (size_t)Np * 1];
+    }
+    EnsOps o{};
+    const std::vector<double> *src[5] = {&D, &L, &Drow, &LiftP, &lf};
+    const double **dst[5] = {&o.D, &o.Lift, &o.Drow, &o.LiftP, &o.lfac};
+    for (int b = 0; b < 5 && !rc; ++b) {
+      double *d = nullptr;
+      if ((rc = ens_alloc(&d, src[b]->size()))) break;
+      s->opbufs.push_back(d);
+      if (hipMemcpy(d, src[b]->data(), src[b]->size() * sizeof(double), hipMemcpyHostToDevice) !=
+          hipSuccess)
+        rc = fail(DGK_ERR_HIP, "operator upload failed");
+      *dst[b] = d;
+    }
+    slot[p[i]] = (int)hops.size();
+    hops.push_back(o);
+  }
+  // instances: geometry constants, offsets, neighbour tables
+  std::vector<int32_t> nbr;
+  long long off = 0;
+  for (int i = 0; i < n && !rc; ++i) {
+    const int Nq = p[i] + 1;
+    EnsInst &I = s->inst[i];
+    I.nq = Nq;
+    I.op = slot[p[i]];
+    I.ne = (long long)nex[i] * ney[i] * nez[i];
+    I.off = off;
+    I.nbr_off = (long long)nbr.size();
+    const double L3[3] = {dom ? dom[3 * i] : 1.0, dom ? dom[3 * i + 1] : 1.0,
+                          dom ? dom[3 * i + 2] : 1.0};
+    const double dl[3] = {L3[0] / nex[i], L3[1] / ney[i], L3[2] / nez[i]};
+    for (int d = 0; d < 3; ++d) I.es[d] = 2.0 / dl[d];                      // mod_mesh.f90:147-149
+    for (int f = 0; f < 6; ++f) I.fs[f] = 2.0 / dl[kFace[f].axis];          // mod_mesh.f90:366-394
+    I.cvel[0] = I.cvel[1] = I.cvel[2] = 0.0;
+    I.dt = 0.0;
+    s->np[i] = Nq * Nq * Nq;
+    off += I.ne * s->np[i];
+    for (int k = 0; k < nez[i]; ++k)
+      for (int j = 0; j < ney[i]; ++j)
+        for (int ii = 0; ii < nex[i]; ++ii)
+          for (int f = 0; f < 6; ++f) {
+            bool wrapped;
+            nbr.push_back((int32_t)neighbour_elem(f, ii, j, k, nex[i], ney[i], nez[i], &wrapped));
+          }
+  }
+  s->total = off;
+  // work items sorted by order, then instance, then element
+  std::vector<int2> items;
+  std::vector<int> order(n);
+  for (int i = 0; i < n; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int b) { return s->inst[a].nq < s->inst[b].nq; });
+  size_t lds = 0;
+  for (int i : order) {
+    const int per = ens_item_elems(s->inst[i].nq);
+    for (long long e = 0; e < s->inst[i].ne; e += per) items.push_back(make_int2(i, (int)e));
+    lds = std::max(lds, ens_item_lds(s->inst[i].nq));
+  }
+  s->nitems = (int)items.size();
+  s->lds = lds;
+  for (size_t t = 0; t < items.size(); ++t) {
+    const int nq = s->inst[items[t].x].nq;
+    if (!s->ncls || s->cls_nq[s->ncls - 1] != nq) {
+      s->cls_nq[s->ncls] = nq;
+      s->cls_item[s->ncls] = (int)t;
+      s->cls_weight[s->ncls] = 0.0;
+      ++s->ncls;
+    }
+  }
+  s->cls_item[s->ncls] = (int)items.size();
+  for (int i = 0; i < n; ++i)
+    for (int c = 0; c < s->ncls; ++c)
+      if (s->cls_nq[c] == s->inst[i].nq) s->cls_weight[c] += (double)s->inst[i].ne * s->np[i];
+  if (!rc) rc = ens_alloc(&s->q[0], off);
+  if (!rc) rc = ens_alloc(&s->q[1], off);
+  if (!rc) rc = ens_alloc(&s->q0, off);
+  if (!rc) rc = ens_alloc(&s->u, off);
+  if (!rc) rc = ens_alloc(&s->v, off);
+  if (!rc) rc = ens_alloc(&s->w, off);
+  if (!rc) rc = ens_alloc(&s->nbr, nbr.size());
+  if (!rc) rc = ens_alloc(&s->dinst, (size_t)n);
+  if (!rc) rc = ens_alloc(&s->dops, hops.size());
+  if (!rc) rc = ens_alloc(&s->ditems, items.size());
+  if (!rc && (hipMemcpy(s->nbr, nbr.data(), nbr.size() * sizeof(int32_t), hipMemcpyHostToDevice) ||
+              hipMemcpy(s->dops, hops.data(), hops.size() * sizeof(EnsOps), hipMemcpyHostToDevice) ||
+              hipMemcpy(s->ditems, items.data(), items.size() * sizeof(int2),
+                        hipMemcpyHostToDevice) ||
+              hipMemcpy(s->dinst, s->inst.data(), n * sizeof(EnsInst), hipMemcpyHostToDevice)))
+    rc = fail(DGK_ERR_HIP, "ensemble table upload failed");
+  if (!rc && hipMemset(s->q0, 0, off * sizeof(double)) != hipSuccess)
+    rc = fail(DGK_ERR_HIP, "hipMemset failed");
+  if (!rc) {
+    s->fn = pick_ens((flags & DGK_FLAG_FMA) != 0, (flags & DGK_FLAG_FACTORED_LIFT) != 0,
+                     (flags & DGK_FLAG_CONST_VELOCITY) != 0);
+    int dev = 0, ncu = 0, per_cu = 0;
+    if (hipFuncSetAttribute((const void *)s->fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)s->lds) != hipSuccess ||
+        hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)s->fn, kEnsThreads,
+                                                     s->lds) != hipSuccess)
+      rc = fail(DGK_ERR_HIP, "ensemble kernel configuration failed");
+    s->slots = std::max(1, per_cu) * ncu / kNumXcd * kNumXcd;
+    ens_partition(s);
+  }
+  if (rc) {
+    dgk_ensemble_destroy(s);
+    return rc;
+  }
+  *out = s;
+  return DGK_OK;
+}
+
+int dgk_ensemble_set_fields(dgk_ensemble *s, int i, const double *q, const double *u,
+                            const double *v, const double *w) {
+  if (!s || i < 0 || i >= s->n || !q || !u || !v || !w) return fail(DGK_ERR_ARG, "bad argument");
+  EnsInst &I = s->inst[i];
+  const size_t cnt = (size_t)I.ne * s->np[i];
+  if (s->flags & DGK_FLAG_CONST_VELOCITY) {
+    const double *vel[3] = {u, v, w};
+    for (int d = 0; d < 3; ++d) {
+      for (size_t t = 1; t < cnt; ++t)
+        if (vel[d][t] != vel[d][0])
+          return fail(DGK_ERR_ARG, "instance %d: velocity component %d is not uniform", i, d + 1);
+      I.cvel[d] = vel[d][0];
+    }
+    HIPOK(hipMemcpy(s->dinst + i, &I, sizeof(EnsInst), hipMemcpyHostToDevice));
+  }
+  double *dst[4] = {s->q[s->cur] + I.off, s->u + I.off, s->v + I.off, s->w + I.off};
+  const double *src[4] = {q, u, v, w};
+  for (int b = 0; b < 4; ++b)
+    HIPOK(hipMemcpy(dst[b], src[b], cnt * sizeof(double), hipMemcpyHostToDevice));
+  s->fields_set = true;
+  return DGK_OK;
+}
+
+int dgk_ensemble_get_field(dgk_ensemble *s, int i, double *q) {
+  if (!s || i < 0 || i >= s->n || !q) return fail(DGK_ERR_ARG, "bad argument");
+  HIPOK(hipStreamSynchronize(s->stream));
+  const EnsInst &I = s->inst[i];
+  HIPOK(hipMemcpy(q, s->q[s->cur] + I.off, (size_t)I.ne * s->np[i] * sizeof(double),
+                  hipMemcpyDeviceToHost));
+  return DGK_OK;
+}
+
+int dgk_ensemble_set_dt(dgk_ensemble *s, const double *dt) {
+  if (!s || !dt) return fail(DGK_ERR_ARG, "bad argument");
+  HIPOK(hipStreamSynchronize(s->stream));
+  for (int i = 0; i < s->n; ++i) s->inst[i].dt = dt[i];
+  HIPOK(hipMemcpy(s->dinst, s->inst.data(), s->n * sizeof(EnsInst), hipMemcpyHostToDevice));
+  return DGK_OK;
+}
+
+int dgk_ensemble_step(dgk_ensemble *s) {
+  if (!s) return fail(DGK_ERR_ARG, "null ensemble");
+  if (!s->fields_set) return fail(DGK_ERR_STATE, "no fields set");
+  const int c0 = s->cur;
+  if (!s->graph[c0]) {
+    hipGraph_t g;
+    HIPOK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+    int rc = DGK_OK;
+    for (int st = 0; st < 3 && !rc; ++st) rc = ens_launch(s, st);
+    hipError_t ce = hipStreamEndCapture(s->stream, &g);
+    s->cur = c0;
+    if (rc) return rc;
+    if (ce != hipSuccess) return fail(DGK_ERR_HIP, "graph capture: %s", hipGetErrorString(ce));
+    hipError_t ie = hipGraphInstantiate(&s->graph[c0], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ie != hipSuccess) return fail(DGK_ERR_HIP, "graph instantiate: %s", hipGetErrorString(ie));
+  }
+  HIPOK(hipGraphLaunch(s->graph[c0], s->stream));
+  s->cur = c0 ^ 1;
+  return DGK_OK;
+}
+
+int dgk_ensemble_run(dgk_ensemble *s, int nstep) {
+  for (int it = 0; it < nstep; ++it) {
+    int rc = dgk_ensemble_step(s);
+    if (rc) return rc;
+  }
+  HIPOK(hipStreamSynchronize(s->stream));
+  return DGK_OK;
+}
+
+int dgk_ensemble_time_stages(dgk_ensemble *s, int nstage, double *ms_per_stage) {
+  if (!s || nstage < 1 || !ms_per_stage) return fail(DGK_ERR_ARG, "bad argument");
+  if (!s->fields_set) return fail(DGK_ERR_STATE, "no fields set");
+  hipEvent_t e0, e1;
+  HIPOK(hipEventCreate(&e0));
+  HIPOK(hipEventCreate(&e1));
+  HIPOK(hipEventRecord(e0, s->stream));
+  for (int it = 0; it < nstage; ++it) {
+    int rc = ens_launch(s, it % 3);
+    if (rc) return rc;
+  }
+  HIPOK(hipEventRecord(e1, s->stream));
+  HIPOK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIPOK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms_per_stage = ms / nstage;
+  return DGK_OK;
+}
+
+int dgk_ensemble_tune(dgk_ensemble *s, int nrep, double *ms_per_class) {
+  if (!s || nrep < 1) return fail(DGK_ERR_ARG, "bad argument");
+  if (!s->fields_set) return fail(DGK_ERR_STATE, "no fields set");
+  HIPOK(hipStreamSynchronize(s->stream));
+  hipEvent_t e0, e1;
+  HIPOK(hipEventCreate(&e0));
+  HIPOK(hipEventCreate(&e1));
+  int rc = DGK_OK;
+  const int units = std::max(s->slots / kNumXcd, 1);
+  for (int c = 0; c < s->ncls && !rc; ++c) {
+    // stage 1 of class c alone on the whole grid: reads q, writes q0 (= q,
+    // as the next step's first stage does) and the unused ping-pong buffer,
+    // so the state is left as it was
+    EnsArgs A = ens_args(s, 0);
+    const int u = std::min(units, ens_unit_cap(s, c));
+    A.ncls = 1;
+    A.cls_nq[0] = s->cls_nq[c];
+    A.cls_item[0] = s->cls_item[c];
+    A.cls_item[1] = s->cls_item[c + 1];
+    A.cls_block[0] = 0;
+    A.cls_block[1] = kNumXcd * u;
+    if ((rc = ens_launch_args(s, A))) break;
+    if (hipEventRecord(e0, s->stream) != hipSuccess) rc = fail(DGK_ERR_HIP, "hipEventRecord");
+    for (int r = 0; r < nrep && !rc; ++r) rc = ens_launch_args(s, A);
+    float ms = 0.f;
+    if (!rc && (hipEventRecord(e1, s->stream) != hipSuccess ||
+                hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+      rc = fail(DGK_ERR_HIP, "event timing failed");
+    if (rc) break;
+    const double t = (double)ms / nrep;
+    if (ms_per_class) ms_per_class[c] = t;
+    // workgroups needed for an equal finish time scale with the time the
+    // class takes alone times the workgroups it had then
+    s->cls_weight[c] = t * u;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  ens_drop_graphs(s);
+  ens_partition(s);
+  s->tuned = true;
+  return DGK_OK;
+}
+
+int dgk_ensemble_query(dgk_ensemble *s, int what, int64_t *value) {
+  if (!s || !value) return fail(DGK_ERR_ARG, "bad argument");
+  switch (what) {
+  case 0: *value = s->total; break;          // DOF over all instances
+  case 1: *value = s->nitems; break;         // work items per stage
+  case 2: *value = s->grid; break;           // workgroups of the persistent launch
+  case 3: *value = (int64_t)s->lds; break;   // LDS bytes per workgroup
+  case 4: *value = s->ncls; break;           // classes of equal order
+  case 5: *value = s->tuned; break;          // grid split from measured times
+  default:
+    if (what >= 16 && what < 16 + s->ncls) {  // Nq of class c = what-16
+      *value = s->cls_nq[what - 16];
+    } else if (what >= 32 && what < 32 + s->ncls) {  // workgroups of class c
+      *value = s->cls_block[what - 32 + 1] - s->cls_block[what - 32];
+    } else {
+      return fail(DGK_ERR_ARG, "unknown query %d", what);
+    }
+  }
+  return DGK_OK;
+}
+
+}  // extern "C"

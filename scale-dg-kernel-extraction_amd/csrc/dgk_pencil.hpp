@@ -119,45 +119,24 @@ __device__ __forceinline__ void st(double *p, double v) {
 // CV: constant velocity (P.cvel): only q and q0 stream through HBM; the
 // products q*u etc. and the face velocities are those of uniform u,v,w
 // fields, so the results are the same bits.
+// One element (or EPW elements of one wavefront) of the pencil stage.  e is
+// this lane's element or -1 (idle lane), e_any any element of the same
+// wavefront (for the address math of idle lanes), blk the LDS block of this
+// lane's element, pl the lane's line index a + NQ*b.  Every lane of the
+// workgroup must call it (it holds two workgroup barriers).
 template <int NQ, int ORDER, bool FMA, bool FACT_LIFT, bool NT, bool CV>
-__global__ void __launch_bounds__(64)
-    pencil_kernel(const StageParams P, const double *__restrict__ Drow,
-                  const double *__restrict__ Lift, const double *__restrict__ lfac) {
+__device__ __forceinline__ void pencil_element(const StageParams &P, const double *__restrict__ Drow,
+                                               const double *__restrict__ Lift,
+                                               const double *__restrict__ lfac, double *blk,
+                                               int pl, long long e, long long e_any) {
   // Lift: the pair-packed copy described at the lift loads below
   using S = PencilShape<NQ>;
-  constexpr int NP = S::NP, NFP = S::NFP, EPW = S::EPW, PAD = S::PAD, FPAD = S::FPAD;
-  extern __shared__ double smem[];
-  const int lane = threadIdx.x;
-  const int el = lane / NFP;
-  const int pl = lane - el * NFP;  // line index a + NQ*b
+  constexpr int NP = S::NP, NFP = S::NFP, PAD = S::PAD, FPAD = S::FPAD;
   const int la = pl % NQ, lb = pl / NQ;
-  double *Bq = smem + el * S::PER_ELEM;
+  double *Bq = blk;
   double *Bu = Bq + PAD;  // u, then D_x
   double *Bv = Bu + PAD;  // v, then D_y
   double *Bf = Bv + PAD;  // fluxes of faces -y, +y, -x, +x
-
-  // XCD-aware element order: workgroup b runs on XCD b % 8.  Linear order:
-  // XCD x walks the x-th eighth of the element groups in order.  Tiled order
-  // (structured meshes): see tiled_element.
-  const int xcd = blockIdx.x % kNumXcd;
-  const long long g = blockIdx.x / kNumXcd;
-  long long e, e_any;
-  if (P.tnx) {
-    if (g * EPW >= P.tseq) return;  // uniform over the workgroup
-    e = tiled_element(P, xcd, g * EPW + el);
-    // any valid element of this workgroup, for the address math of idle lanes
-    e_any = e;
-    for (int t = 0; t < EPW && e_any < 0; ++t) e_any = tiled_element(P, xcd, g * EPW + t);
-    if (e_any < 0) return;  // uniform: no lane of this workgroup has an element
-  } else {
-    const long long ngroups = (P.Ne + EPW - 1) / EPW;
-    const long long per = (ngroups + kNumXcd - 1) / kNumXcd;
-    const long long gl = (long long)xcd * per + g;
-    if (gl >= ngroups) return;  // uniform over the workgroup
-    e = gl * EPW + el;
-    if (e >= P.Ne) e = -1;
-    e_any = gl * EPW;
-  }
   const bool valid = e >= 0;
   const long long eb = valid ? e : e_any;  // a valid element for address math
 
@@ -313,6 +292,43 @@ __global__ void __launch_bounds__(64)
       st<NT>(P.qout + idx, qn);
     }
   }
+}
+
+template <int NQ, int ORDER, bool FMA, bool FACT_LIFT, bool NT, bool CV>
+__global__ void __launch_bounds__(64)
+    pencil_kernel(const StageParams P, const double *__restrict__ Drow,
+                  const double *__restrict__ Lift, const double *__restrict__ lfac) {
+  using S = PencilShape<NQ>;
+  constexpr int NFP = S::NFP, EPW = S::EPW;
+  extern __shared__ double smem[];
+  const int lane = threadIdx.x;
+  const int el = lane / NFP;
+  const int pl = lane - el * NFP;  // line index a + NQ*b
+
+  // XCD-aware element order: workgroup b runs on XCD b % 8.  Linear order:
+  // XCD x walks the x-th eighth of the element groups in order.  Tiled order
+  // (structured meshes): see tiled_element.
+  const int xcd = blockIdx.x % kNumXcd;
+  const long long g = blockIdx.x / kNumXcd;
+  long long e, e_any;
+  if (P.tnx) {
+    if (g * EPW >= P.tseq) return;  // uniform over the workgroup
+    e = tiled_element(P, xcd, g * EPW + el);
+    // any valid element of this workgroup, for the address math of idle lanes
+    e_any = e;
+    for (int t = 0; t < EPW && e_any < 0; ++t) e_any = tiled_element(P, xcd, g * EPW + t);
+    if (e_any < 0) return;  // uniform: no lane of this workgroup has an element
+  } else {
+    const long long ngroups = (P.Ne + EPW - 1) / EPW;
+    const long long per = (ngroups + kNumXcd - 1) / kNumXcd;
+    const long long gl = (long long)xcd * per + g;
+    if (gl >= ngroups) return;  // uniform over the workgroup
+    e = gl * EPW + el;
+    if (e >= P.Ne) e = -1;
+    e_any = gl * EPW;
+  }
+  pencil_element<NQ, ORDER, FMA, FACT_LIFT, NT, CV>(P, Drow, Lift, lfac,
+                                                     smem + el * S::PER_ELEM, pl, e, e_any);
 }
 
 }  // namespace dgk

@@ -33,7 +33,10 @@ EXPORTS = (
     "dgk_destroy", "dgk_set_fields", "dgk_get_field", "dgk_field_ptr", "dgk_cal_tend",
     "dgk_rk_stage", "dgk_step", "dgk_run", "dgk_synchronize", "dgk_query",
     "dgk_time_stages", "dgk_num_variants", "dgk_set_variant", "dgk_advect3d_eq_cal_tend", "dgk_divlike_dirXYZ_batched",
-    "dgk_lift_hexahedral_batched",
+    "dgk_lift_hexahedral_batched", "dgk_ensemble_create", "dgk_ensemble_destroy",
+    "dgk_ensemble_set_fields", "dgk_ensemble_get_field", "dgk_ensemble_set_dt",
+    "dgk_ensemble_step", "dgk_ensemble_run", "dgk_ensemble_time_stages",
+    "dgk_ensemble_tune", "dgk_ensemble_query",
 )
 
 
@@ -82,6 +85,16 @@ def lib():
         "dgk_advect3d_eq_cal_tend": ([vp] * 13 + [i] * 6, i),
         "dgk_divlike_dirXYZ_batched": ([i, i, i] + [vp] * 9, i),
         "dgk_lift_hexahedral_batched": ([i, i, vp, vp, vp, vp], i),
+        "dgk_ensemble_create": ([vp, i] + [vp] * 5 + [C.c_char_p, i, i], i),
+        "dgk_ensemble_destroy": ([vp], i),
+        "dgk_ensemble_set_fields": ([vp, i] + [vp] * 4, i),
+        "dgk_ensemble_get_field": ([vp, i, vp], i),
+        "dgk_ensemble_set_dt": ([vp, vp], i),
+        "dgk_ensemble_step": ([vp], i),
+        "dgk_ensemble_run": ([vp, i], i),
+        "dgk_ensemble_time_stages": ([vp, i, vp], i),
+        "dgk_ensemble_tune": ([vp, i, vp], i),
+        "dgk_ensemble_query": ([vp, i, vp], i),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -290,3 +303,84 @@ def divlike_batched(Nq, nelem, typeid, Mat, Mat_tr, in_x, in_y, in_z, out_x, out
 
 def lift_batched(Nq, nelem, Lift, fin, fout, stream=None):
     _check(lib().dgk_lift_hexahedral_batched(Nq, nelem, Lift, fin, fout, stream))
+
+
+class Ensemble:
+    """Many independent periodic runs advanced together (dgk_ensemble_*).
+
+    ``specs`` is a sequence of dicts with keys ``p`` and ``ne`` = (NeX,NeY,NeZ)
+    and optionally ``dom`` (domain lengths, default unit cube).  Every
+    instance evolves exactly as a ``Solver.structured`` run with the same
+    flags in OPT1's summation order."""
+
+    def __init__(self, specs, flags=0, typeid=TYPE_HIP, opdir: str = OPERATOR_DIR):
+        self.specs = [dict(s) for s in specs]
+        n = len(self.specs)
+        p = np.array([s["p"] for s in self.specs], dtype=np.int32)
+        ne = np.array([s["ne"] for s in self.specs], dtype=np.int32).reshape(n, 3)
+        nex, ney, nez = (np.ascontiguousarray(ne[:, d]) for d in range(3))
+        dom = np.array([s.get("dom", (1.0, 1.0, 1.0)) for s in self.specs],
+                       dtype=np.float64).reshape(n, 3)
+        dom = np.ascontiguousarray(dom).ravel()  # n triples
+        h = C.c_void_p()
+        _check(lib().dgk_ensemble_create(C.byref(h), n, _p(p), _p(nex), _p(ney), _p(nez),
+                                         _p(dom), opdir.encode(), typeid, flags))
+        self._h = h
+        self.n = n
+        self.shapes = [((s["p"] + 1) ** 3, int(np.prod(s["ne"]))) for s in self.specs]
+
+    def close(self):
+        if self._h:
+            lib().dgk_ensemble_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_fields(self, i, q, u, v, w):
+        arrs = [np.asfortranarray(a, dtype=np.float64) for a in (q, u, v, w)]
+        for a in arrs:
+            if a.shape != self.shapes[i]:
+                raise ValueError(f"instance {i}: expected shape {self.shapes[i]}, got {a.shape}")
+        _check(lib().dgk_ensemble_set_fields(self._h, i, *[_p(a) for a in arrs]))
+
+    def get_field(self, i):
+        q = np.empty(self.shapes[i], order="F")
+        _check(lib().dgk_ensemble_get_field(self._h, i, _p(q)))
+        return q
+
+    def set_dt(self, dt):
+        dt = np.ascontiguousarray(dt, dtype=np.float64)
+        if dt.shape != (self.n,):
+            raise ValueError("one dt per instance")
+        _check(lib().dgk_ensemble_set_dt(self._h, _p(dt)))
+
+    def step(self):
+        _check(lib().dgk_ensemble_step(self._h))
+
+    def run(self, nstep):
+        _check(lib().dgk_ensemble_run(self._h, nstep))
+
+    def time_stages(self, nstage):
+        ms = C.c_double(0.0)
+        _check(lib().dgk_ensemble_time_stages(self._h, nstage, C.byref(ms)))
+        return ms.value
+
+    def tune(self, nrep=3):
+        """Per-class stage time in ms (classes by increasing order)."""
+        ms = np.zeros(self.query(4))
+        _check(lib().dgk_ensemble_tune(self._h, nrep, _p(ms)))
+        return ms
+
+    def query(self, what):
+        v = C.c_longlong(0)
+        _check(lib().dgk_ensemble_query(self._h, what, C.byref(v)))
+        return v.value
+
+    @property
+    def classes(self):
+        """[(Nq, workgroups)] of the persistent grid."""
+        return [(self.query(16 + c), self.query(32 + c)) for c in range(self.query(4))]

@@ -89,7 +89,36 @@ WORKLOADS = {
     "c2_32cube_p7": dict(ne=(32, 32, 32), p=7, vel=(1.0, -0.7, 0.3), ic="sine4", seed=1002, nstep=100),
     "c3_64cube_p7": dict(ne=(64, 64, 64), p=7, vel=(1.0, -0.7, 0.3), ic="sine4", seed=1002, nstep=100),
     "c5_16cube_p15": dict(ne=(16, 16, 16), p=15, vel=(0.8, 0.6, -0.4), ic="sine8", seed=1005, nstep=50),
+    # single-instance timing points for the orders of the ensemble workload
+    "p11_24cube": dict(ne=(24, 24, 24), p=11, vel=(0.6, -0.5, 0.7), ic="sine4", seed=1011, nstep=20),
+    "p5_32cube": dict(ne=(32, 32, 32), p=5, vel=(0.6, -0.5, 0.7), ic="sine4", seed=1015, nstep=20),
+    "p3_64cube": dict(ne=(64, 64, 64), p=3, vel=(0.6, -0.5, 0.7), ic="sine4", seed=1013, nstep=20),
 }
+
+# The ensemble workload: 64 instances drawn by ensemble_workload(64, 1004).
+ENSEMBLE = dict(n=64, seed=1004, nstep=50)
+
+
+def ensemble_workload(n: int = 64, seed: int = 1004, orders=(3, 5, 7, 11),
+                      ne_choices=(8, 12, 16, 24), cfl: float = 0.1):
+    """Seeded ensemble of periodic unit-cube instances: per instance the order
+    from ``orders``, each axis's element count from ``ne_choices``, a constant
+    velocity with components U[-1,1], a Gaussian blob with centre U[0,1]^3 and
+    width U[0.05,0.2], and dt at the given CFL.  Draws per instance, in this
+    order: order, NeX, NeY, NeZ, u, v, w, centre x, y, z, width."""
+    g = SplitMix64(seed)
+    out = []
+    for _ in range(n):
+        r = g.uniform(11)
+        p = orders[min(int(r[0] * len(orders)), len(orders) - 1)]
+        ne = tuple(ne_choices[min(int(r[1 + d] * len(ne_choices)), len(ne_choices) - 1)]
+                   for d in range(3))
+        vel = tuple(float(2.0 * r[4 + d] - 1.0) for d in range(3))
+        centre = tuple(float(r[7 + d]) for d in range(3))
+        width = float(0.05 + 0.15 * r[10])
+        out.append(dict(p=p, ne=ne, vel=vel, centre=centre, width=width,
+                        dt=cfl_dt(p, vel, ne, cfl=cfl)))
+    return out
 
 
 def initial_field(kind: str, pos, seed: int):

@@ -1,0 +1,222 @@
+"""The batched ensemble (dgk_ensemble_*): many periodic instances of uneven
+order, mesh and velocity advanced by one persistent launch per RK stage.
+
+Every instance must give the bits of the oracle's run of that instance alone
+(OPT1 summation order) and of a single-instance device solver; the factored
+lift, the constant-velocity form, the tuned grid split and the graph replay
+change the schedule or the rounding only where stated.
+"""
+import numpy as np
+import pytest
+
+import dgk
+import oracle as O
+from conftest import say
+from dgk import ic
+
+pytestmark = pytest.mark.gpu
+
+# Uneven small instances: every supported order, ragged work items (element
+# counts that are not multiples of the elements per item), mixed velocities.
+SMALL = [
+    dict(p=1, ne=(3, 4, 5)),
+    dict(p=3, ne=(4, 3, 5)),
+    dict(p=5, ne=(3, 3, 4)),
+    dict(p=7, ne=(3, 5, 3)),
+    dict(p=11, ne=(2, 3, 2)),
+    dict(p=3, ne=(2, 2, 3)),
+    dict(p=7, ne=(2, 2, 2)),
+]
+
+
+def _setup(specs, seed):
+    g = ic.SplitMix64(seed)
+    out = []
+    for s in specs:
+        r = g.uniform(7)
+        vel = tuple(float(2.0 * r[d] - 1.0) for d in range(3))
+        centre = tuple(float(r[3 + d]) for d in range(3))
+        width = float(0.05 + 0.15 * r[6])
+        x1D, D, L = dgk.read_operator(s["p"])
+        m = dgk.mesh_setup(*s["ne"], s["p"], x1D=x1D, tables=False)
+        q = ic.blob(m["pos_en"], centre, width) + 0.01 * np.sin(6.0 * m["pos_en"][:, :, 0])
+        q = np.asfortranarray(q)
+        out.append(dict(s, vel=vel, q=q, D=D, L=L,
+                        dt=ic.cfl_dt(s["p"], vel, s["ne"])))
+    return out
+
+
+def _ensemble(inst, flags=0):
+    ens = dgk.Ensemble(inst, flags=flags)
+    for i, s in enumerate(inst):
+        ens.set_fields(i, s["q"], *(np.full_like(s["q"], c) for c in s["vel"]))
+    ens.set_dt([s["dt"] for s in inst])
+    return ens
+
+
+def _oracle(s, nstep):
+    m = O.Mesh(*s["ne"], s["p"])
+    F = []
+    for a in (s["q"], *(np.full_like(s["q"], c) for c in s["vel"])):
+        f = m.field()
+        f[:, :m.Ne] = a
+        m.update_halo(f)
+        F.append(f)
+    m.run(*F, s["dt"], nstep, O.OPT1)
+    return F[0][:, :m.Ne]
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.fixture(scope="module")
+def small():
+    return _setup(SMALL, 4242)
+
+
+def test_ensemble_matches_oracle_bitwise(gpu, small):
+    nstep = 3
+    say("[ens] exact, 3 steps")
+    ens = _ensemble(small)
+    assert ens.query(4) == 5  # Nq 2, 4, 6, 8, 12
+    ens.run(nstep)
+    for i, s in enumerate(small):
+        want = _oracle(s, nstep)
+        got = ens.get_field(i)
+        assert np.array_equal(got, want), (i, s["p"], s["ne"], rel_l2(got, want))
+    ens.close()
+
+
+def test_ensemble_matches_single_solver(gpu, small):
+    """Each instance equals a single-instance device run with the same flags
+    (FMA contractions included)."""
+    nstep = 2
+    for flags in (0, dgk.FLAG_FMA):
+        say(f"[ens] vs Solver flags={flags}")
+        ens = _ensemble(small, flags)
+        ens.run(nstep)
+        for i, s in enumerate(small):
+            sv = dgk.Solver.structured(*s["ne"], s["p"], s["D"], s["L"], flags=flags)
+            sv.set_fields(s["q"], *(np.full_like(s["q"], c) for c in s["vel"]))
+            sv.run(s["dt"], nstep)
+            assert np.array_equal(ens.get_field(i), sv.get_field()), (i, flags)
+            sv.close()
+        ens.close()
+
+
+def test_ensemble_const_velocity_same_bits(gpu, small):
+    say("[ens] constant-velocity form")
+    a = _ensemble(small)
+    b = _ensemble(small, dgk.FLAG_CONST_VELOCITY)
+    a.run(2)
+    b.run(2)
+    for i in range(len(small)):
+        assert np.array_equal(a.get_field(i), b.get_field(i)), i
+    a.close()
+    b.close()
+
+
+def test_ensemble_factored_lift(gpu, small):
+    """The 1D-factored lift is exact to rounding: within 1e-12 of the oracle
+    per instance after 3 steps."""
+    say("[ens] factored lift")
+    ens = _ensemble(small, dgk.FLAG_FACTORED_LIFT | dgk.FLAG_CONST_VELOCITY)
+    ens.run(3)
+    for i, s in enumerate(small):
+        assert rel_l2(ens.get_field(i), _oracle(s, 3)) <= 1e-12, i
+    ens.close()
+
+
+def test_ensemble_tune_keeps_state_and_bits(gpu, small):
+    say("[ens] tune")
+    a = _ensemble(small)
+    b = _ensemble(small)
+    ms = b.tune(2)
+    assert ms.shape == (5,) and (ms > 0).all()
+    assert b.query(5) == 1
+    cls = b.classes
+    assert [c[0] for c in cls] == [2, 4, 6, 8, 12]
+    assert all(wg > 0 and wg % 8 == 0 for _, wg in cls)
+    for i in range(len(small)):  # tuning left the state alone
+        assert np.array_equal(b.get_field(i), small[i]["q"])
+    a.run(2)
+    b.run(2)
+    for i in range(len(small)):
+        assert np.array_equal(a.get_field(i), b.get_field(i)), i
+    a.close()
+    b.close()
+
+
+def test_ensemble_step_graph_vs_run(gpu, small):
+    """Graph replays (step) and the stage-timing launches advance the same
+    state: 3 stages of time_stages equal one step."""
+    say("[ens] step vs time_stages")
+    a = _ensemble(small)
+    b = _ensemble(small)
+    for _ in range(2):
+        a.step()
+    b.time_stages(6)
+    for i in range(len(small)):
+        assert np.array_equal(a.get_field(i), b.get_field(i)), i
+    a.close()
+    b.close()
+
+
+def test_ensemble_errors(gpu):
+    with pytest.raises(dgk.DGKError) as e:
+        dgk.Ensemble([dict(p=2, ne=(2, 2, 2))])
+    assert e.value.code == dgk.ERR_ORDER
+    inst = _setup([dict(p=3, ne=(2, 2, 2))], 7)
+    ens = dgk.Ensemble(inst)
+    with pytest.raises(dgk.DGKError) as e:
+        ens.step()
+    assert e.value.code == dgk.ERR_STATE
+    with pytest.raises(ValueError):
+        ens.set_fields(0, inst[0]["q"][:, :3], *([inst[0]["q"]] * 3))
+    ens.close()
+    cv = dgk.Ensemble(inst, flags=dgk.FLAG_CONST_VELOCITY)
+    q = inst[0]["q"]
+    u = np.full_like(q, 0.5)
+    u[3, 2] = 0.25
+    with pytest.raises(dgk.DGKError) as e:
+        cv.set_fields(0, q, u, u, u)
+    assert e.value.code == dgk.ERR_ARG
+    cv.close()
+
+
+def test_ensemble_workload_properties(gpu):
+    """The 64-instance workload (150 M DOF), 5 steps: finite, mass conserved
+    per instance, and p=3 instances bit-exact against the oracle."""
+    inst = ic.ensemble_workload(**{k: v for k, v in ic.ENSEMBLE.items() if k in ("n", "seed")})
+    say(f"[ens] workload, {len(inst)} instances")
+    fields = []
+    for s in inst:
+        x1D, _, _ = dgk.read_operator(s["p"])
+        m = dgk.mesh_setup(*s["ne"], s["p"], x1D=x1D, tables=False)
+        fields.append((ic.blob(m["pos_en"], s["centre"], s["width"]), x1D))
+    ens = dgk.Ensemble(inst, flags=dgk.FLAG_CONST_VELOCITY)
+    for i, s in enumerate(inst):
+        q = fields[i][0]
+        ens.set_fields(i, q, *(np.full_like(q, c) for c in s["vel"]))
+    ens.set_dt([s["dt"] for s in inst])
+    ens.tune(2)
+    ens.run(5)
+    checked = 0
+    for i, s in enumerate(inst):
+        got = ens.get_field(i)
+        assert np.isfinite(got).all(), i
+        x1D = fields[i][1]
+        pp = len(x1D) - 1
+        P = np.polynomial.legendre.Legendre.basis(pp)(x1D)
+        w = 2.0 / (pp * (pp + 1) * P * P)
+        w3 = np.einsum("i,j,k->ijk", w, w, w).ravel(order="F")
+        q0 = fields[i][0]
+        scale = float(np.dot(w3, np.abs(q0).sum(axis=1)))
+        assert abs(float(np.dot(w3, got.sum(axis=1) - q0.sum(axis=1)))) <= 1e-13 * scale, i
+        if s["p"] == 3 and checked < 2:
+            want = _oracle(dict(s, q=np.asfortranarray(q0)), 5)
+            assert np.array_equal(got, want), i
+            checked += 1
+    assert checked == 2
+    ens.close()
