@@ -103,7 +103,8 @@ struct StageKernel {
   LaunchFn launch = nullptr;
   int nt = 0;
   size_t lds = 0;
-  int epb = 1;       // elements per workgroup (node kernel) or wavefront (pencil)
+  int epb = 1;       // elements per workgroup (node kernel, Nq 6/12/16 pencil) or per
+                     // wavefront (Nq 2/4/8 pencil)
   bool pencil = false;
   long long grid = 0;
 };
@@ -146,36 +147,88 @@ StageKernel make_pencil() {
   return k;
 }
 
-// Runtime choices to template arguments, one level at a time.
-template <int NQ, int ORDER, bool FMA, bool FACT, bool NT>
-StageKernel pick_pencil_cv(bool cv) {
-  return cv ? make_pencil<NQ, ORDER, FMA, FACT, NT, true>()
-            : make_pencil<NQ, ORDER, FMA, FACT, NT, false>();
-}
-template <int NQ, int ORDER, bool FMA, bool FACT>
-StageKernel pick_pencil_nt(bool nt, bool cv) {
-  return nt ? pick_pencil_cv<NQ, ORDER, FMA, FACT, true>(cv)
-            : pick_pencil_cv<NQ, ORDER, FMA, FACT, false>(cv);
-}
-template <int NQ, int ORDER, bool FMA>
-StageKernel pick_pencil_fact(bool fact, bool nt, bool cv) {
-  return fact ? pick_pencil_nt<NQ, ORDER, FMA, true>(nt, cv)
-              : pick_pencil_nt<NQ, ORDER, FMA, false>(nt, cv);
-}
+// Multi-wavefront pencil workgroups (Nq 6, 12, 16): at most MAXTH threads,
+// epb elements per workgroup chosen at run time.  Nq 12 and 16 need about
+// 210 and 256 VGPRs, so their workgroups stay at five and four wavefronts
+// (at most two per SIMD, which allows 256 VGPRs).
 template <int NQ>
-StageKernel pick_pencil_nq(int order, bool fma, bool fact, bool nt, bool cv) {
-  if (order == kGeneral)
-    return fma ? pick_pencil_fact<NQ, kGeneral, true>(fact, nt, cv)
-               : pick_pencil_fact<NQ, kGeneral, false>(fact, nt, cv);
-  return fma ? pick_pencil_fact<NQ, kOpt1, true>(fact, nt, cv)
-             : pick_pencil_fact<NQ, kOpt1, false>(fact, nt, cv);
+constexpr int group_maxth() { return NQ == 6 ? 576 : (NQ == 12 ? 320 : 256); }
+
+template <int NQ, int ORDER, bool FMA, bool FACT, bool NT, bool CV>
+void launch_pencil_group(const StageKernel &k, const StageParams &P, const HostOps &H,
+                         const double *Lift, hipStream_t st) {
+  (void)Lift;
+  hipLaunchKernelGGL((pencil_group_kernel<NQ, group_maxth<NQ>(), ORDER, FMA, FACT, NT, CV>),
+                     dim3((unsigned)k.grid), dim3(k.nt), k.lds, st, P, H.Drow, H.LiftP, H.lfac,
+                     k.epb);
 }
 
-StageKernel pick_pencil(int Nq, int order, bool fma, bool fact, bool nt, bool cv) {
+// Elements per workgroup: the narrow shape or the wide one (Nq 6: 4 or 9
+// wavefronts, Nq 12: 3 or 5; Nq 16 has one shape).
+template <int NQ>
+int group_epb(bool wide) {
+  if constexpr (NQ == 6) return wide ? 16 : 7;
+  else if constexpr (NQ == 12) return wide ? 2 : 1;
+  else return 1;
+}
+
+template <int NQ, int ORDER, bool FMA, bool FACT, bool NT, bool CV>
+StageKernel make_pencil_group(bool wide) {
+  StageKernel k;
+  k.kernel = (const void *)pencil_group_kernel<NQ, group_maxth<NQ>(), ORDER, FMA, FACT, NT, CV>;
+  k.launch = launch_pencil_group<NQ, ORDER, FMA, FACT, NT, CV>;
+  k.epb = group_epb<NQ>(wide);
+  k.nt = pencil_group_threads<NQ>(k.epb);
+  k.lds = pencil_group_lds<NQ>(k.epb, CV);
+  k.pencil = true;
+  return k;
+}
+
+// Runtime choices to template arguments, one level at a time.
+template <int NQ, int ORDER, bool FMA, bool FACT, bool NT>
+StageKernel pick_pencil_cv(bool cv, bool wide) {
+  if constexpr (NQ == 6 || NQ == 12 || NQ == 16)
+    return cv ? make_pencil_group<NQ, ORDER, FMA, FACT, NT, true>(wide)
+              : make_pencil_group<NQ, ORDER, FMA, FACT, NT, false>(wide);
+  else
+    return cv ? make_pencil<NQ, ORDER, FMA, FACT, NT, true>()
+              : make_pencil<NQ, ORDER, FMA, FACT, NT, false>();
+}
+template <int NQ, int ORDER, bool FMA, bool FACT>
+StageKernel pick_pencil_nt(bool nt, bool cv, bool wide) {
+  // the multi-wavefront kernels are built with non-temporal stores only
+  if constexpr (NQ == 6 || NQ == 12 || NQ == 16)
+    return pick_pencil_cv<NQ, ORDER, FMA, FACT, true>(cv, wide);
+  else
+    return nt ? pick_pencil_cv<NQ, ORDER, FMA, FACT, true>(cv, wide)
+              : pick_pencil_cv<NQ, ORDER, FMA, FACT, false>(cv, wide);
+}
+template <int NQ, int ORDER, bool FMA>
+StageKernel pick_pencil_fact(bool fact, bool nt, bool cv, bool wide) {
+  return fact ? pick_pencil_nt<NQ, ORDER, FMA, true>(nt, cv, wide)
+              : pick_pencil_nt<NQ, ORDER, FMA, false>(nt, cv, wide);
+}
+template <int NQ>
+StageKernel pick_pencil_nq(int order, bool fma, bool fact, bool nt, bool cv, bool wide) {
+  if (order == kGeneral)
+    return fma ? pick_pencil_fact<NQ, kGeneral, true>(fact, nt, cv, wide)
+               : pick_pencil_fact<NQ, kGeneral, false>(fact, nt, cv, wide);
+  return fma ? pick_pencil_fact<NQ, kOpt1, true>(fact, nt, cv, wide)
+             : pick_pencil_fact<NQ, kOpt1, false>(fact, nt, cv, wide);
+}
+
+bool pencil_nq(int Nq) {
+  return Nq == 2 || Nq == 4 || Nq == 6 || Nq == 8 || Nq == 12 || Nq == 16;
+}
+
+StageKernel pick_pencil(int Nq, int order, bool fma, bool fact, bool nt, bool cv, bool wide) {
   switch (Nq) {
-  case 2: return pick_pencil_nq<2>(order, fma, fact, nt, cv);
-  case 4: return pick_pencil_nq<4>(order, fma, fact, nt, cv);
-  case 8: return pick_pencil_nq<8>(order, fma, fact, nt, cv);
+  case 2: return pick_pencil_nq<2>(order, fma, fact, nt, cv, wide);
+  case 4: return pick_pencil_nq<4>(order, fma, fact, nt, cv, wide);
+  case 6: return pick_pencil_nq<6>(order, fma, fact, nt, cv, wide);
+  case 8: return pick_pencil_nq<8>(order, fma, fact, nt, cv, wide);
+  case 12: return pick_pencil_nq<12>(order, fma, fact, nt, cv, wide);
+  case 16: return pick_pencil_nq<16>(order, fma, fact, nt, cv, wide);
   default: return StageKernel{};
   }
 }
@@ -289,22 +342,30 @@ int default_pencil_variant(int Nq) {
   return kPencilNtStore;
 }
 
+// Tuning variants of the solver's stage kernel: store flavour x element
+// order, x workgroup shape for the multi-wavefront pencil (Nq 6, 12, 16).
+int pencil_variants(const dgk_solver *s) {
+  if (!s->sk.pencil) return 1;
+  return (s->Nq == 6 || s->Nq == 12) ? kPencilVariants : kPencilWide;
+}
+
 int configure_launch(dgk_solver *s) {
-  const bool use_pencil = s->structured && !(s->flags & DGK_FLAG_NODE_KERNEL) &&
-                          (s->Nq == 2 || s->Nq == 4 || s->Nq == 8);
+  const bool use_pencil =
+      s->structured && !(s->flags & DGK_FLAG_NODE_KERNEL) && pencil_nq(s->Nq);
   const int var = s->variant >= 0 ? s->variant : default_pencil_variant(s->Nq);
   if ((s->flags & DGK_FLAG_CONST_VELOCITY) && !use_pencil)
-    return fail(DGK_ERR_ARG, "DGK_FLAG_CONST_VELOCITY needs the pencil kernel (Nq 2, 4 or 8, "
-                             "compressed connectivity)");
+    return fail(DGK_ERR_ARG, "DGK_FLAG_CONST_VELOCITY needs the pencil kernel (Nq 2, 4, 6, 8, "
+                             "12 or 16, compressed connectivity)");
   if (use_pencil)
     s->sk = pick_pencil(s->Nq, s->order, s->fma, (s->flags & DGK_FLAG_FACTORED_LIFT) != 0,
-                        (var & kPencilNtStore) != 0, (s->flags & DGK_FLAG_CONST_VELOCITY) != 0);
+                        (var & kPencilNtStore) != 0, (s->flags & DGK_FLAG_CONST_VELOCITY) != 0,
+                        (var & kPencilWide) != 0);
   else
     s->sk = pick_stage(s->Nq, s->order, s->fma, s->structured);
   if (!s->sk.kernel) return fail(DGK_ERR_ORDER, "Nq %d not supported", s->Nq);
   if ((s->flags & DGK_FLAG_FACTORED_LIFT) && !use_pencil)
-    return fail(DGK_ERR_ARG, "DGK_FLAG_FACTORED_LIFT needs the pencil kernel (Nq 2, 4 or 8, "
-                             "compressed connectivity)");
+    return fail(DGK_ERR_ARG, "DGK_FLAG_FACTORED_LIFT needs the pencil kernel (Nq 2, 4, 6, 8, "
+                             "12 or 16, compressed connectivity)");
   HIPOK(hipFuncSetAttribute(s->sk.kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)s->sk.lds));
   const long long ngroups = (s->Ne + s->sk.epb - 1) / s->sk.epb;
@@ -833,7 +894,7 @@ int dgk_query(dgk_solver *s, int what, int64_t *value) {
 
 int dgk_num_variants(dgk_solver *s, int *n) {
   if (!s || !n) return fail(DGK_ERR_ARG, "null argument");
-  *n = s->sk.pencil ? kPencilVariants : 1;  // template variants x {tiled, linear order}
+  *n = pencil_variants(s);
   return DGK_OK;
 }
 
@@ -843,8 +904,8 @@ int dgk_set_variant(dgk_solver *s, int variant) {
     if (variant != 0) return fail(DGK_ERR_ARG, "this stage kernel has one variant");
     return DGK_OK;
   }
-  if (variant < -1 || variant >= kPencilVariants)
-    return fail(DGK_ERR_ARG, "variant %d outside -1..%d", variant, kPencilVariants - 1);
+  if (variant < -1 || variant >= pencil_variants(s))
+    return fail(DGK_ERR_ARG, "variant %d outside -1..%d", variant, pencil_variants(s) - 1);
   s->variant = variant;
   drop_graphs(s);
   return configure_launch(s);

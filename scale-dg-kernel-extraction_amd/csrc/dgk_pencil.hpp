@@ -30,15 +30,29 @@ namespace dgk {
 
 template <int NQ>
 struct PencilShape {
-  static_assert(NQ == 2 || NQ == 4 || NQ == 8, "pencil kernel: Nq^2 must divide 64");
+  static_assert(NQ == 2 || NQ == 4 || NQ == 6 || NQ == 8 || NQ == 12 || NQ == 16,
+                "pencil stage: Nq in {2, 4, 6, 8, 12, 16}");
   static constexpr int NP = NQ * NQ * NQ;
   static constexpr int NFP = NQ * NQ;
-  static constexpr int EPW = 64 / NFP;         // elements per wavefront
+  static constexpr int EPW = NFP <= 64 ? 64 / NFP : 1;  // elements per wavefront (Nq 2, 4, 8)
   static constexpr int PAD = NP + NP / NQ;     // padded volume buffer
   static constexpr int FPAD = NFP + NFP / NQ;  // padded face buffer
   static constexpr int PER_ELEM = 3 * PAD + 4 * FPAD;
   static constexpr size_t LDS = sizeof(double) * EPW * PER_ELEM;
+  // LDS of one element in the lean layout: q, the partial sum S and (for
+  // non-uniform velocity) u and v share three volume buffers, or two.
+  static constexpr int lean_elem(bool cv) { return (cv ? 2 : 3) * PAD + 4 * FPAD; }
 };
+
+// Threads and LDS of a multi-wavefront pencil workgroup of epb elements:
+// Nq^2 lanes each, rounded up to whole wavefronts (the lanes beyond
+// epb*Nq^2 only take part in the barriers).
+template <int NQ>
+constexpr int pencil_group_threads(int epb) { return (epb * NQ * NQ + 63) / 64 * 64; }
+template <int NQ>
+constexpr size_t pencil_group_lds(int epb, bool cv) {
+  return sizeof(double) * epb * PencilShape<NQ>::lean_elem(cv);
+}
 
 template <int NQ>
 __device__ __forceinline__ int pidx(int n) { return n + n / NQ; }
@@ -49,42 +63,77 @@ __device__ __forceinline__ int pidx(int n) { return n + n / NQ; }
 // coefficients at the top of the kernel and spills them to VGPR lanes, which
 // costs more VALU issue slots than the sweeps themselves.
 // `after` is a value the row must not be loaded before (the previous
-// output): it keeps one row of SGPRs live at a time.
+// output): it keeps one row of SGPRs live at a time.  Rows longer than 16
+// dwords take two loads; their destinations are early-clobber so that the
+// first cannot overwrite the base address the second still reads.
 template <int NQ>
 __device__ __forceinline__ void sgpr_row(const double *Drow, int k, double after,
                                          double (&r)[NQ]) {
   const double *p = Drow + k * NQ;
-  if constexpr (NQ == 8) {
-    typedef int v16 __attribute__((ext_vector_type(16)));
+  constexpr int NDW = 2 * NQ;  // dwords per row
+  typedef int v16 __attribute__((ext_vector_type(16)));
+  typedef int v8 __attribute__((ext_vector_type(8)));
+  typedef int v4 __attribute__((ext_vector_type(4)));
+  int dw[NDW];
+  if constexpr (NQ == 16) {
+    v16 x, y;
+    asm volatile("s_load_dwordx16 %0, %2, 0x0\n\ts_load_dwordx16 %1, %2, 0x40\n\t"
+                 "s_waitcnt lgkmcnt(0)"
+                 : "=&s"(x), "=&s"(y)
+                 : "s"(p), "v"(after));
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+      dw[t] = x[t];
+      dw[16 + t] = y[t];
+    }
+  } else if constexpr (NQ == 12) {
+    v16 x;
+    v8 y;
+    asm volatile("s_load_dwordx16 %0, %2, 0x0\n\ts_load_dwordx8 %1, %2, 0x40\n\t"
+                 "s_waitcnt lgkmcnt(0)"
+                 : "=&s"(x), "=&s"(y)
+                 : "s"(p), "v"(after));
+#pragma unroll
+    for (int t = 0; t < 16; ++t) dw[t] = x[t];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) dw[16 + t] = y[t];
+  } else if constexpr (NQ == 8) {
     v16 x;
     asm volatile("s_load_dwordx16 %0, %1, 0x0\n\ts_waitcnt lgkmcnt(0)"
                  : "=s"(x)
                  : "s"(p), "v"(after));
 #pragma unroll
-    for (int l = 0; l < 8; ++l)
-      r[l] = __builtin_bit_cast(double, (unsigned long long)(unsigned)x[2 * l] |
-                                            ((unsigned long long)(unsigned)x[2 * l + 1] << 32));
+    for (int t = 0; t < 16; ++t) dw[t] = x[t];
+  } else if constexpr (NQ == 6) {
+    v8 x;
+    v4 y;
+    asm volatile("s_load_dwordx8 %0, %2, 0x0\n\ts_load_dwordx4 %1, %2, 0x20\n\t"
+                 "s_waitcnt lgkmcnt(0)"
+                 : "=&s"(x), "=&s"(y)
+                 : "s"(p), "v"(after));
+#pragma unroll
+    for (int t = 0; t < 8; ++t) dw[t] = x[t];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) dw[8 + t] = y[t];
   } else if constexpr (NQ == 4) {
-    typedef int v8 __attribute__((ext_vector_type(8)));
     v8 x;
     asm volatile("s_load_dwordx8 %0, %1, 0x0\n\ts_waitcnt lgkmcnt(0)"
                  : "=s"(x)
                  : "s"(p), "v"(after));
 #pragma unroll
-    for (int l = 0; l < 4; ++l)
-      r[l] = __builtin_bit_cast(double, (unsigned long long)(unsigned)x[2 * l] |
-                                            ((unsigned long long)(unsigned)x[2 * l + 1] << 32));
+    for (int t = 0; t < 8; ++t) dw[t] = x[t];
   } else {
-    typedef int v4 __attribute__((ext_vector_type(4)));
     v4 x;
     asm volatile("s_load_dwordx4 %0, %1, 0x0\n\ts_waitcnt lgkmcnt(0)"
                  : "=s"(x)
                  : "s"(p), "v"(after));
 #pragma unroll
-    for (int l = 0; l < 2; ++l)
-      r[l] = __builtin_bit_cast(double, (unsigned long long)(unsigned)x[2 * l] |
-                                            ((unsigned long long)(unsigned)x[2 * l + 1] << 32));
+    for (int t = 0; t < 4; ++t) dw[t] = x[t];
   }
+#pragma unroll
+  for (int l = 0; l < NQ; ++l)
+    r[l] = __builtin_bit_cast(double, (unsigned long long)(unsigned)dw[2 * l] |
+                                          ((unsigned long long)(unsigned)dw[2 * l + 1] << 32));
 }
 
 // out[k] = sum_l D(k,l) f[l] in the reference's order, one SGPR row at a time.
@@ -106,9 +155,12 @@ __device__ __forceinline__ void sweep(const double *Drow, const double (&f)[NQ],
 //   kPencilLinearOrder  linear element order instead of the tiled order on
 //                       structured meshes (a launch parameter, not a
 //                       template argument)
+//   kPencilWide         Nq 6, 12: the wide workgroup shape (more elements
+//                       per workgroup)
 constexpr int kPencilNtStore = 1;
 constexpr int kPencilLinearOrder = 2;
-constexpr int kPencilVariants = 4;
+constexpr int kPencilWide = 4;
+constexpr int kPencilVariants = 8;
 
 template <bool NT>
 __device__ __forceinline__ void st(double *p, double v) {
@@ -119,24 +171,32 @@ __device__ __forceinline__ void st(double *p, double v) {
 // CV: constant velocity (P.cvel): only q and q0 stream through HBM; the
 // products q*u etc. and the face velocities are those of uniform u,v,w
 // fields, so the results are the same bits.
-// One element (or EPW elements of one wavefront) of the pencil stage.  e is
-// this lane's element or -1 (idle lane), e_any any element of the same
-// wavefront (for the address math of idle lanes), blk the LDS block of this
-// lane's element, pl the lane's line index a + NQ*b.  Every lane of the
-// workgroup must call it (it holds two workgroup barriers).
-template <int NQ, int ORDER, bool FMA, bool FACT_LIFT, bool NT, bool CV>
+// One element's lines (or EPW elements of one wavefront) of the pencil stage.
+// e is this lane's element or -1 (idle lane), e_any any element of the same
+// workgroup (for the address math of idle lanes), blk the LDS block of this
+// lane's element, pl the lane's line index a + NQ*b.  act = false marks a
+// lane beyond the workgroup's elements: it writes no LDS (blk may then be
+// any block).  Every lane of the workgroup must call it (it holds two
+// workgroup barriers, three when LEAN).
+// LEAN: the x phase runs first and leaves Es1*D_x in a buffer S that the y
+// phase turns into Es1*D_x + Es2*D_y (the reference's summation order), so
+// that with uniform velocity two volume buffers suffice instead of three.
+template <int NQ, int ORDER, bool FMA, bool FACT_LIFT, bool NT, bool CV, bool LEAN = false>
 __device__ __forceinline__ void pencil_element(const StageParams &P, const double *__restrict__ Drow,
                                                const double *__restrict__ Lift,
                                                const double *__restrict__ lfac, double *blk,
-                                               int pl, long long e, long long e_any) {
+                                               int pl, long long e, long long e_any,
+                                               bool act = true) {
   // Lift: the pair-packed copy described at the lift loads below
   using S = PencilShape<NQ>;
   constexpr int NP = S::NP, NFP = S::NFP, PAD = S::PAD, FPAD = S::FPAD;
+  constexpr int NVOL = (LEAN && CV) ? 2 : 3;
   const int la = pl % NQ, lb = pl / NQ;
   double *Bq = blk;
-  double *Bu = Bq + PAD;  // u, then D_x
+  double *Bu = Bq + PAD;  // u, then D_x (LEAN: u, then the partial sum S)
   double *Bv = Bu + PAD;  // v, then D_y
-  double *Bf = Bv + PAD;  // fluxes of faces -y, +y, -x, +x
+  double *BS = Bu;        // LEAN: Es1 D_x, then + Es2 D_y
+  double *Bf = Bq + NVOL * PAD;  // fluxes of faces -y, +y, -x, +x
   const bool valid = e >= 0;
   const long long eb = valid ? e : e_any;  // a valid element for address math
 
@@ -186,61 +246,117 @@ __device__ __forceinline__ void pencil_element(const StageParams &P, const doubl
   }
   const double flux4 = upwind_flux(q[0], qP4, -w[0], -wP4, P.fs[4]);
   const double flux5 = upwind_flux(q[NQ - 1], qP5, w[NQ - 1], wP5, P.fs[5]);
+  if (act) {
 #pragma unroll
-  for (int k = 0; k < NQ; ++k) {
-    const int s = pidx<NQ>(pl + NFP * k);
-    Bq[s] = q[k];
-    if constexpr (!CV) {
-      Bu[s] = u[k];
-      Bv[s] = v[k];
+    for (int k = 0; k < NQ; ++k) {
+      const int s = pidx<NQ>(pl + NFP * k);
+      Bq[s] = q[k];
+      if constexpr (!CV) {
+        Bu[s] = u[k];
+        Bv[s] = v[k];
+      }
     }
   }
   __syncthreads();
+  const double e1s = P.es[0], e2s = P.es[1], e3s = P.es[2];
 
-  // ---- y phase: line (i,k) = (la,lb) -------------------------------------
-  {
-    double qy[NQ], vy[NQ], fy[NQ], dy[NQ];
+  if constexpr (LEAN) {
+    // ---- x phase: line (j,k) = (la,lb); S = Es1 D_x ------------------------
+    {
+      double qx[NQ], ux[NQ], fx[NQ], dx[NQ];
 #pragma unroll
-    for (int l = 0; l < NQ; ++l) {
-      const int s = pidx<NQ>(la + NQ * l + NFP * lb);
-      qy[l] = Bq[s];
-      vy[l] = CV ? P.cvel[1] : Bv[s];
-      fy[l] = qy[l] * vy[l];
+      for (int l = 0; l < NQ; ++l) {
+        const int s = pidx<NQ>(l + NQ * pl);
+        qx[l] = Bq[s];
+        ux[l] = CV ? P.cvel[0] : Bu[s];
+        fx[l] = qx[l] * ux[l];
+      }
+      sweep<NQ, ORDER, FMA, true>(Drow, fx, dx);
+      const double f3 = upwind_flux(qx[0], qP3, -ux[0], -uP3, P.fs[3]);
+      const double f1 = upwind_flux(qx[NQ - 1], qP1, ux[NQ - 1], uP1, P.fs[1]);
+      if (act) {
+        Bf[2 * FPAD + pidx<NQ>(pl)] = f3;
+        Bf[3 * FPAD + pidx<NQ>(pl)] = f1;
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) BS[pidx<NQ>(i + NQ * pl)] = e1s * dx[i];
+      }
     }
-    sweep<NQ, ORDER, FMA, false>(Drow, fy, dy);
-    Bf[0 * FPAD + pidx<NQ>(pl)] = upwind_flux(qy[0], qP0, -vy[0], -vP0, P.fs[0]);
-    Bf[1 * FPAD + pidx<NQ>(pl)] = upwind_flux(qy[NQ - 1], qP2, vy[NQ - 1], vP2, P.fs[2]);
+    __syncthreads();
+    // ---- y phase: line (i,k) = (la,lb); S += Es2 D_y -----------------------
+    {
+      double qy[NQ], vy[NQ], fy[NQ], dy[NQ];
 #pragma unroll
-    for (int j = 0; j < NQ; ++j) Bv[pidx<NQ>(la + NQ * j + NFP * lb)] = dy[j];
-  }
+      for (int l = 0; l < NQ; ++l) {
+        const int s = pidx<NQ>(la + NQ * l + NFP * lb);
+        qy[l] = Bq[s];
+        vy[l] = CV ? P.cvel[1] : Bv[s];
+        fy[l] = qy[l] * vy[l];
+      }
+      sweep<NQ, ORDER, FMA, false>(Drow, fy, dy);
+      const double f0 = upwind_flux(qy[0], qP0, -vy[0], -vP0, P.fs[0]);
+      const double f2 = upwind_flux(qy[NQ - 1], qP2, vy[NQ - 1], vP2, P.fs[2]);
+      if (act) {
+        Bf[0 * FPAD + pidx<NQ>(pl)] = f0;
+        Bf[1 * FPAD + pidx<NQ>(pl)] = f2;
+#pragma unroll
+        for (int j = 0; j < NQ; ++j) {
+          const int s = pidx<NQ>(la + NQ * j + NFP * lb);
+          BS[s] = madd<FMA>(e2s, dy[j], BS[s]);
+        }
+      }
+    }
+    __syncthreads();
+  } else {
+    // ---- y phase: line (i,k) = (la,lb) -------------------------------------
+    {
+      double qy[NQ], vy[NQ], fy[NQ], dy[NQ];
+#pragma unroll
+      for (int l = 0; l < NQ; ++l) {
+        const int s = pidx<NQ>(la + NQ * l + NFP * lb);
+        qy[l] = Bq[s];
+        vy[l] = CV ? P.cvel[1] : Bv[s];
+        fy[l] = qy[l] * vy[l];
+      }
+      sweep<NQ, ORDER, FMA, false>(Drow, fy, dy);
+      const double f0 = upwind_flux(qy[0], qP0, -vy[0], -vP0, P.fs[0]);
+      const double f2 = upwind_flux(qy[NQ - 1], qP2, vy[NQ - 1], vP2, P.fs[2]);
+      if (act) {
+        Bf[0 * FPAD + pidx<NQ>(pl)] = f0;
+        Bf[1 * FPAD + pidx<NQ>(pl)] = f2;
+#pragma unroll
+        for (int j = 0; j < NQ; ++j) Bv[pidx<NQ>(la + NQ * j + NFP * lb)] = dy[j];
+      }
+    }
 
-  // ---- x phase: line (j,k) = (la,lb) -------------------------------------
-  {
-    double qx[NQ], ux[NQ], fx[NQ], dx[NQ];
+    // ---- x phase: line (j,k) = (la,lb) -------------------------------------
+    {
+      double qx[NQ], ux[NQ], fx[NQ], dx[NQ];
 #pragma unroll
-    for (int l = 0; l < NQ; ++l) {
-      const int s = pidx<NQ>(l + NQ * pl);
-      qx[l] = Bq[s];
-      ux[l] = CV ? P.cvel[0] : Bu[s];
-      fx[l] = qx[l] * ux[l];
+      for (int l = 0; l < NQ; ++l) {
+        const int s = pidx<NQ>(l + NQ * pl);
+        qx[l] = Bq[s];
+        ux[l] = CV ? P.cvel[0] : Bu[s];
+        fx[l] = qx[l] * ux[l];
+      }
+      sweep<NQ, ORDER, FMA, true>(Drow, fx, dx);
+      const double f3 = upwind_flux(qx[0], qP3, -ux[0], -uP3, P.fs[3]);
+      const double f1 = upwind_flux(qx[NQ - 1], qP1, ux[NQ - 1], uP1, P.fs[1]);
+      if (act) {
+        Bf[2 * FPAD + pidx<NQ>(pl)] = f3;
+        Bf[3 * FPAD + pidx<NQ>(pl)] = f1;
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) Bu[pidx<NQ>(i + NQ * pl)] = dx[i];
+      }
     }
-    sweep<NQ, ORDER, FMA, true>(Drow, fx, dx);
-    Bf[2 * FPAD + pidx<NQ>(pl)] = upwind_flux(qx[0], qP3, -ux[0], -uP3, P.fs[3]);
-    Bf[3 * FPAD + pidx<NQ>(pl)] = upwind_flux(qx[NQ - 1], qP1, ux[NQ - 1], uP1, P.fs[1]);
-#pragma unroll
-    for (int i = 0; i < NQ; ++i) Bu[pidx<NQ>(i + NQ * pl)] = dx[i];
+    __syncthreads();
   }
-  __syncthreads();
 
   // ---- final z phase: lift, combine, RK ----------------------------------
   if (!valid) return;
   const int i = la, j = lb;
-  const double e1s = P.es[0], e2s = P.es[1], e3s = P.es[2];
 #pragma unroll
   for (int k = 0; k < NQ; ++k) {
     const int s = pidx<NQ>(pl + NFP * k);
-    const double dxv = Bu[s];
-    const double dyv = Bv[s];
     const double f0 = Bf[0 * FPAD + pidx<NQ>(i + NQ * k)];  // in(i,k,1)
     const double f2 = Bf[1 * FPAD + pidx<NQ>(i + NQ * k)];  // in(i,k,3)
     const double f3 = Bf[2 * FPAD + pidx<NQ>(j + NQ * k)];  // in(j,k,4)
@@ -275,8 +391,13 @@ __device__ __forceinline__ void pencil_element(const StageParams &P, const doubl
     lift = madd<FMA>(L[4], flux4, lift);
     lift = madd<FMA>(L[5], flux5, lift);
     // dqdt = -(Es1 Dx + Es2 Dy + Es3 Dz + Lift) (mod_advect3d_eq.f90:194-198)
-    double sum = e1s * dxv;
-    sum = madd<FMA>(e2s, dyv, sum);
+    double sum;
+    if constexpr (LEAN) {
+      sum = BS[s];
+    } else {
+      sum = e1s * Bu[s];
+      sum = madd<FMA>(e2s, Bv[s], sum);
+    }
     sum = madd<FMA>(e3s, dz[k], sum);
     const double dq = -(sum + lift);
     const long long idx = base + NFP * k;
@@ -329,6 +450,44 @@ __global__ void __launch_bounds__(64)
   }
   pencil_element<NQ, ORDER, FMA, FACT_LIFT, NT, CV>(P, Drow, Lift, lfac,
                                                      smem + el * S::PER_ELEM, pl, e, e_any);
+}
+
+// Multi-wavefront pencil stage for Nq >= 6: Nq^2 lanes per element, epb
+// elements per workgroup (a launch argument; the workgroup has
+// pencil_group_threads(epb) threads, at most MAXTH), lean LDS layout.
+// Element order as pencil_kernel, in groups of epb.
+template <int NQ, int MAXTH, int ORDER, bool FMA, bool FACT_LIFT, bool NT, bool CV>
+__global__ void __launch_bounds__(MAXTH)
+    pencil_group_kernel(const StageParams P, const double *__restrict__ Drow,
+                        const double *__restrict__ Lift, const double *__restrict__ lfac,
+                        int epb) {
+  constexpr int NFP = NQ * NQ;
+  constexpr int PER = PencilShape<NQ>::lean_elem(CV);
+  extern __shared__ double smem[];
+  const int tid = threadIdx.x;
+  const bool act = tid < epb * NFP;
+  const int el = act ? tid / NFP : 0;
+  const int pl = act ? tid - el * NFP : 0;
+  const int xcd = blockIdx.x % kNumXcd;
+  const long long g = blockIdx.x / kNumXcd;
+  long long e, e_any;
+  if (P.tnx) {
+    if (g * epb >= P.tseq) return;  // uniform over the workgroup
+    e = act ? tiled_element(P, xcd, g * epb + el) : -1;
+    e_any = -1;
+    for (int t = 0; t < epb && e_any < 0; ++t) e_any = tiled_element(P, xcd, g * epb + t);
+    if (e_any < 0) return;  // uniform: no element in this workgroup
+  } else {
+    const long long ngroups = (P.Ne + epb - 1) / epb;
+    const long long per = (ngroups + kNumXcd - 1) / kNumXcd;
+    const long long gl = (long long)xcd * per + g;
+    if (g >= per || gl >= ngroups) return;  // uniform over the workgroup
+    e = gl * epb + el;
+    if (!act || e >= P.Ne) e = -1;
+    e_any = gl * epb;
+  }
+  pencil_element<NQ, ORDER, FMA, FACT_LIFT, NT, CV, true>(P, Drow, Lift, lfac, smem + el * PER,
+                                                          pl, e, e_any, act);
 }
 
 }  // namespace dgk

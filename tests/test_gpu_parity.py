@@ -72,10 +72,14 @@ def test_batched_operators_bit_exact(gpu, golden, p, tname):
 # tendency (advect3d_eq_cal_tend)
 
 
+# orders with a pencil stage kernel (Nq 2, 4, 8 one wavefront; 6, 12, 16
+# several)
+PENCIL_NQ = (2, 4, 6, 8, 12, 16)
+
 def _solver_variants(ne, p, D, L, typeid, flags=0):
     m = dgk.mesh_setup(*ne, p)
     s = dgk.Solver.structured(*ne, p, D, L, typeid=typeid, flags=flags)
-    assert s.kernel_kind == ("pencil" if p + 1 in (2, 4, 8) else "node")
+    assert s.kernel_kind == ("pencil" if p + 1 in PENCIL_NQ else "node")
     yield "structured-" + s.kernel_kind, s
     if p + 1 in (2, 4, 8):
         s = dgk.Solver.structured(*ne, p, D, L, typeid=typeid, flags=flags | dgk.FLAG_NODE_KERNEL)
@@ -144,14 +148,14 @@ FMA_VARIANTS = [("fma", dgk.FLAG_FMA), ("fma-node", dgk.FLAG_FMA | dgk.FLAG_NODE
                 ("factored", dgk.FLAG_FACTORED_LIFT)]
 
 
-@pytest.mark.parametrize("case", CS.RUN_CASES[:3], ids=lambda c: CS.case_key(*c[0], c[1], c[2]))
+@pytest.mark.parametrize("case", CS.RUN_CASES, ids=lambda c: CS.case_key(*c[0], c[1], c[2]))
 @pytest.mark.parametrize("variant", FMA_VARIANTS, ids=lambda v: v[0])
 def test_trajectory_fma_tolerance(gpu, case, variant):
     """FMA contractions and/or the factored lift: rel-L2 <= 1e-12 after one
     step, <= 1e-10 after nstep against the oracle (OPT1 order)."""
     ne, p, nstep, vel, kind = case
     name, flags = variant
-    if (flags & dgk.FLAG_FACTORED_LIFT) and p + 1 not in (2, 4, 8):
+    if (flags & dgk.FLAG_FACTORED_LIFT) and p + 1 not in PENCIL_NQ:
         pytest.skip("the factored lift exists in the pencil kernel only")
     _, D, L = dgk.read_operator(p)
     om = O.Mesh(*ne, p)
@@ -244,7 +248,7 @@ def test_stretched_mesh_keeps_tables(gpu):
     s.close()
 
 
-@pytest.mark.parametrize("p", [1, 3, 7])
+@pytest.mark.parametrize("p", [1, 3, 5, 7, 11, 15])
 @pytest.mark.parametrize("ne", [(3, 2, 9), (4, 5, 3)])
 def test_every_tuning_variant_is_bit_exact(gpu, p, ne):
     """All code shapes the autotuner may pick (tiled and linear element
@@ -269,7 +273,7 @@ def test_every_tuning_variant_is_bit_exact(gpu, p, ne):
     s.close()
 
 
-@pytest.mark.parametrize("p", [1, 3, 7])
+@pytest.mark.parametrize("p", [1, 3, 5, 7, 11, 15])
 @pytest.mark.parametrize("factored", [False, True])
 def test_constant_velocity_form(gpu, p, factored):
     """DGK_FLAG_CONST_VELOCITY streams no velocity and gives the same bits as
