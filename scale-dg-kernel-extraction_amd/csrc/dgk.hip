@@ -4,6 +4,7 @@
 #include "dgk_internal.hpp"
 #include "dgk_kernels.hpp"
 #include "dgk_pencil.hpp"
+#include "dgk_launch.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -85,30 +86,6 @@ namespace {
 
 using StageFn = void (*)(const StageParams);
 
-// Device copies of the 1D operator in row-major order (D(k,l) at
-// [k*Nq + l], read a row at a time by the pencil kernel) and of the vectors
-// [l_-, l_+] of the factored lift.
-struct HostOps {
-  const double *Drow;
-  const double *lfac;
-  const double *LiftP;  // pair-packed Lift for the pencil kernel
-};
-
-struct StageKernel;
-using LaunchFn = void (*)(const StageKernel &, const StageParams &, const HostOps &,
-                          const double *Lift, hipStream_t);
-
-struct StageKernel {
-  const void *kernel = nullptr;  // for attributes and occupancy
-  LaunchFn launch = nullptr;
-  int nt = 0;
-  size_t lds = 0;
-  int epb = 1;       // elements per workgroup (node kernel, Nq 6/12/16 pencil) or per
-                     // wavefront (Nq 2/4/8 pencil)
-  bool pencil = false;
-  long long grid = 0;
-};
-
 template <int NQ, int ORDER, bool FMA, bool STRUCT>
 void launch_node(const StageKernel &k, const StageParams &P, const HostOps &, const double *,
                  hipStream_t st) {
@@ -147,88 +124,56 @@ StageKernel make_pencil() {
   return k;
 }
 
-// Multi-wavefront pencil workgroups (Nq 6, 12, 16): at most MAXTH threads,
-// epb elements per workgroup chosen at run time.  Nq 12 and 16 need about
-// 210 and 256 VGPRs, so their workgroups stay at five and four wavefronts
-// (at most two per SIMD, which allows 256 VGPRs).
-template <int NQ>
-constexpr int group_maxth() { return NQ == 6 ? 576 : (NQ == 12 ? 320 : 256); }
-
-template <int NQ, int ORDER, bool FMA, bool FACT, bool NT, bool CV>
-void launch_pencil_group(const StageKernel &k, const StageParams &P, const HostOps &H,
-                         const double *Lift, hipStream_t st) {
-  (void)Lift;
-  hipLaunchKernelGGL((pencil_group_kernel<NQ, group_maxth<NQ>(), ORDER, FMA, FACT, NT, CV>),
-                     dim3((unsigned)k.grid), dim3(k.nt), k.lds, st, P, H.Drow, H.LiftP, H.lfac,
-                     k.epb);
-}
-
-// Elements per workgroup: the narrow shape or the wide one (Nq 6: 4 or 9
-// wavefronts, Nq 12: 3 or 5; Nq 16 has one shape).
-template <int NQ>
-int group_epb(bool wide) {
-  if constexpr (NQ == 6) return wide ? 16 : 7;
-  else if constexpr (NQ == 12) return wide ? 2 : 1;
-  else return 1;
-}
-
-template <int NQ, int ORDER, bool FMA, bool FACT, bool NT, bool CV>
-StageKernel make_pencil_group(bool wide) {
-  StageKernel k;
-  k.kernel = (const void *)pencil_group_kernel<NQ, group_maxth<NQ>(), ORDER, FMA, FACT, NT, CV>;
-  k.launch = launch_pencil_group<NQ, ORDER, FMA, FACT, NT, CV>;
-  k.epb = group_epb<NQ>(wide);
-  k.nt = pencil_group_threads<NQ>(k.epb);
-  k.lds = pencil_group_lds<NQ>(k.epb, CV);
-  k.pencil = true;
-  return k;
+// Elements per workgroup of the multi-wavefront pencil: Nq 2, 4, 8 (wide
+// variant only) four wavefronts; Nq 6 narrow 7 elements (four wavefronts) or
+// wide 16 (nine); Nq 12 narrow 1 (three) or wide 2 (five); Nq 16 one.
+int group_epb(int Nq, bool wide) {
+  switch (Nq) {
+  case 6: return wide ? 16 : 7;
+  case 12: return wide ? 2 : 1;
+  case 16: return 1;
+  default: return 256 / (Nq * Nq);
+  }
 }
 
 // Runtime choices to template arguments, one level at a time.
 template <int NQ, int ORDER, bool FMA, bool FACT, bool NT>
-StageKernel pick_pencil_cv(bool cv, bool wide) {
-  if constexpr (NQ == 6 || NQ == 12 || NQ == 16)
-    return cv ? make_pencil_group<NQ, ORDER, FMA, FACT, NT, true>(wide)
-              : make_pencil_group<NQ, ORDER, FMA, FACT, NT, false>(wide);
-  else
-    return cv ? make_pencil<NQ, ORDER, FMA, FACT, NT, true>()
-              : make_pencil<NQ, ORDER, FMA, FACT, NT, false>();
+StageKernel pick_pencil_cv(bool cv) {
+  return cv ? make_pencil<NQ, ORDER, FMA, FACT, NT, true>()
+            : make_pencil<NQ, ORDER, FMA, FACT, NT, false>();
 }
 template <int NQ, int ORDER, bool FMA, bool FACT>
-StageKernel pick_pencil_nt(bool nt, bool cv, bool wide) {
-  // the multi-wavefront kernels are built with non-temporal stores only
-  if constexpr (NQ == 6 || NQ == 12 || NQ == 16)
-    return pick_pencil_cv<NQ, ORDER, FMA, FACT, true>(cv, wide);
-  else
-    return nt ? pick_pencil_cv<NQ, ORDER, FMA, FACT, true>(cv, wide)
-              : pick_pencil_cv<NQ, ORDER, FMA, FACT, false>(cv, wide);
+StageKernel pick_pencil_nt(bool nt, bool cv) {
+  return nt ? pick_pencil_cv<NQ, ORDER, FMA, FACT, true>(cv)
+            : pick_pencil_cv<NQ, ORDER, FMA, FACT, false>(cv);
 }
 template <int NQ, int ORDER, bool FMA>
-StageKernel pick_pencil_fact(bool fact, bool nt, bool cv, bool wide) {
-  return fact ? pick_pencil_nt<NQ, ORDER, FMA, true>(nt, cv, wide)
-              : pick_pencil_nt<NQ, ORDER, FMA, false>(nt, cv, wide);
+StageKernel pick_pencil_fact(bool fact, bool nt, bool cv) {
+  return fact ? pick_pencil_nt<NQ, ORDER, FMA, true>(nt, cv)
+              : pick_pencil_nt<NQ, ORDER, FMA, false>(nt, cv);
 }
 template <int NQ>
-StageKernel pick_pencil_nq(int order, bool fma, bool fact, bool nt, bool cv, bool wide) {
+StageKernel pick_pencil_nq(int order, bool fma, bool fact, bool nt, bool cv) {
   if (order == kGeneral)
-    return fma ? pick_pencil_fact<NQ, kGeneral, true>(fact, nt, cv, wide)
-               : pick_pencil_fact<NQ, kGeneral, false>(fact, nt, cv, wide);
-  return fma ? pick_pencil_fact<NQ, kOpt1, true>(fact, nt, cv, wide)
-             : pick_pencil_fact<NQ, kOpt1, false>(fact, nt, cv, wide);
+    return fma ? pick_pencil_fact<NQ, kGeneral, true>(fact, nt, cv)
+               : pick_pencil_fact<NQ, kGeneral, false>(fact, nt, cv);
+  return fma ? pick_pencil_fact<NQ, kOpt1, true>(fact, nt, cv)
+             : pick_pencil_fact<NQ, kOpt1, false>(fact, nt, cv);
 }
 
 bool pencil_nq(int Nq) {
   return Nq == 2 || Nq == 4 || Nq == 6 || Nq == 8 || Nq == 12 || Nq == 16;
 }
 
+// Nq 2, 4, 8: the one-wavefront pencil_kernel, or with `wide` the
+// four-wavefront lean group kernel; Nq 6, 12, 16: the group kernel.
 StageKernel pick_pencil(int Nq, int order, bool fma, bool fact, bool nt, bool cv, bool wide) {
+  if (Nq == 6 || Nq == 12 || Nq == 16 || wide)
+    return pick_pencil_group(Nq, order, fma, fact, cv, group_epb(Nq, wide));
   switch (Nq) {
-  case 2: return pick_pencil_nq<2>(order, fma, fact, nt, cv, wide);
-  case 4: return pick_pencil_nq<4>(order, fma, fact, nt, cv, wide);
-  case 6: return pick_pencil_nq<6>(order, fma, fact, nt, cv, wide);
-  case 8: return pick_pencil_nq<8>(order, fma, fact, nt, cv, wide);
-  case 12: return pick_pencil_nq<12>(order, fma, fact, nt, cv, wide);
-  case 16: return pick_pencil_nq<16>(order, fma, fact, nt, cv, wide);
+  case 2: return pick_pencil_nq<2>(order, fma, fact, nt, cv);
+  case 4: return pick_pencil_nq<4>(order, fma, fact, nt, cv);
+  case 8: return pick_pencil_nq<8>(order, fma, fact, nt, cv);
   default: return StageKernel{};
   }
 }
@@ -343,10 +288,10 @@ int default_pencil_variant(int Nq) {
 }
 
 // Tuning variants of the solver's stage kernel: store flavour x element
-// order, x workgroup shape for the multi-wavefront pencil (Nq 6, 12, 16).
+// order x workgroup shape (Nq 16 has one shape).
 int pencil_variants(const dgk_solver *s) {
   if (!s->sk.pencil) return 1;
-  return (s->Nq == 6 || s->Nq == 12) ? kPencilVariants : kPencilWide;
+  return s->Nq == 16 ? kPencilWide : kPencilVariants;
 }
 
 int configure_launch(dgk_solver *s) {

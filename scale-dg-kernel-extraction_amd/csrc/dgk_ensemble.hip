@@ -7,9 +7,9 @@
 // into classes of equal order; the persistent grid is split between the
 // classes (in units of eight workgroups, one per XCD) and every workgroup
 // runs items of one class only, so that it is compiled and scheduled as one
-// straight code path.  A workgroup of 256 threads runs an item as
-//   Nq = 2, 4, 8   four wavefronts, each the pencil stage of its element(s);
-//   Nq = 6, 12     the cooperative thread-per-node stage of one element.
+// straight code path.  A workgroup of 256 threads runs an item as the pencil
+// stage (one lane per node line) of 64, 16, 7, 4 or 1 elements for Nq = 2,
+// 4, 6, 8, 12.
 // Within a class the items are split into eight contiguous ranges, one per
 // XCD, so that an instance's neighbouring elements stay in one L2.  The
 // split of the grid between the classes follows measured per-class stage
@@ -69,24 +69,22 @@ struct EnsArgs {
 
 bool ens_supported(int nq) { return nq == 2 || nq == 4 || nq == 6 || nq == 8 || nq == 12; }
 
-// elements per item for a given Nq
-int ens_item_elems(int nq) {
-  switch (nq) {
-  case 2: return 4 * PencilShape<2>::EPW;
-  case 4: return 4 * PencilShape<4>::EPW;
-  case 8: return 4 * PencilShape<8>::EPW;
-  case 6: return GroupShape<6, kEnsThreads>::EPB;
-  default: return GroupShape<12, kEnsThreads>::EPB;
-  }
+// Elements per item (one workgroup): Nq^2 lanes per element, 64, 16, 7, 4
+// and 1 elements for Nq 2, 4, 6, 8, 12.
+int ens_item_elems(int nq) { return kEnsThreads / (nq * nq); }
+
+template <int NQ>
+size_t ens_lds_nq(bool cv) {
+  return sizeof(double) * (kEnsThreads / (NQ * NQ)) * PencilShape<NQ>::lean_elem(cv);
 }
 
-size_t ens_item_lds(int nq) {
+size_t ens_item_lds(int nq, bool cv) {
   switch (nq) {
-  case 2: return 4 * PencilShape<2>::LDS;
-  case 4: return 4 * PencilShape<4>::LDS;
-  case 8: return 4 * PencilShape<8>::LDS;
-  case 6: return GroupShape<6, kEnsThreads>::LDS;
-  default: return GroupShape<12, kEnsThreads>::LDS;
+  case 2: return ens_lds_nq<2>(cv);
+  case 4: return ens_lds_nq<4>(cv);
+  case 6: return ens_lds_nq<6>(cv);
+  case 8: return ens_lds_nq<8>(cv);
+  default: return ens_lds_nq<12>(cv);
   }
 }
 
@@ -101,17 +99,35 @@ __device__ __forceinline__ T *uniform_ptr(T *p) {
   return (T *)(((unsigned long long)hi << 32) | lo);
 }
 
+// *p for a uniform address, returned in SGPRs: the tables are read with
+// vector loads (the compiler cannot prove them constant) and every dword is
+// made scalar, which keeps the instance constants out of the VGPR budget.
+template <class T>
+__device__ __forceinline__ T uniform_load(const T *p) {
+  static_assert(sizeof(T) % 4 == 0, "dword-sized records");
+  constexpr int N = sizeof(T) / 4;
+  const unsigned *w = reinterpret_cast<const unsigned *>(p);
+  unsigned v[N];
+#pragma unroll
+  for (int t = 0; t < N; ++t) v[t] = __builtin_amdgcn_readfirstlane(w[t]);
+  return __builtin_bit_cast(T, v);
+}
+
+// One item: the pencil stage (lean LDS layout) of the elements first ..
+// first+EPB-1 of an instance; lanes beyond EPB*Nq^2 only join the barriers.
 template <int NQ, int ORDER, bool FMA, bool FACT, bool CV>
 __device__ __forceinline__ void ens_pencil(const StageParams &P, const EnsOps &O, long long first,
                                            double *smem) {
-  using S = PencilShape<NQ>;
-  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-  const int el = lane / S::NFP, pl = lane - el * S::NFP;
-  long long e = first + (long long)wave * S::EPW + el;
-  if (e >= P.Ne) e = -1;
-  pencil_element<NQ, ORDER, FMA, FACT, true, CV>(P, uniform_ptr(O.Drow), O.LiftP, O.lfac,
-                                                  smem + (wave * S::EPW + el) * S::PER_ELEM, pl,
-                                                  e, first);
+  constexpr int NFP = NQ * NQ, EPB = kEnsThreads / NFP;
+  constexpr int PER = PencilShape<NQ>::lean_elem(CV);
+  const int tid = threadIdx.x;
+  const bool act = tid < EPB * NFP;
+  const int el = act ? tid / NFP : 0;
+  const int pl = act ? tid - el * NFP : 0;
+  long long e = first + el;
+  if (!act || e >= P.Ne) e = -1;
+  pencil_element<NQ, ORDER, FMA, FACT, true, CV, true>(P, uniform_ptr(O.Drow), O.LiftP, O.lfac,
+                                                        smem + el * PER, pl, e, first, act);
 }
 
 // The items [first, first+count) of one class on nb workgroups (lb = this
@@ -123,9 +139,9 @@ __device__ __forceinline__ void ens_items(const EnsArgs &A, int first, int count
   const int per = (count + kNumXcd - 1) / kNumXcd;
   const int end = first + min(count, (xcd + 1) * per);
   for (int it = first + xcd * per + slot; it < end; it += nslot) {
-    const int2 item = A.items[it];
-    const EnsInst &I = A.inst[item.x];
-    const EnsOps O = A.ops[I.op];
+    const int2 item = uniform_load(A.items + it);
+    const EnsInst I = uniform_load(A.inst + item.x);
+    const EnsOps O = uniform_load(A.ops + I.op);
     StageParams P{};
     P.qin = A.qin + I.off;
     P.qout = A.qout + I.off;
@@ -148,11 +164,7 @@ __device__ __forceinline__ void ens_items(const EnsArgs &A, int first, int count
     P.b = A.b;
     P.dt = I.dt;
     P.mode = A.mode;
-    if constexpr (NQ == 2 || NQ == 4 || NQ == 8)
-      ens_pencil<NQ, ORDER, FMA, FACT, CV>(P, O, item.y, smem);
-    else
-      node_group<NQ, kEnsThreads, ORDER, FMA, FACT, CV>(
-          P, O.lfac, I.fs, I.cvel, item.y / GroupShape<NQ, kEnsThreads>::EPB, threadIdx.x, smem);
+    ens_pencil<NQ, ORDER, FMA, FACT, CV>(P, O, item.y, smem);
     __syncthreads();  // LDS is reused by the next item
   }
 }
@@ -439,7 +451,7 @@ int dgk_ensemble_create(dgk_ensemble **out, int n, const int32_t *p, const int32
   for (int i : order) {
     const int per = ens_item_elems(s->inst[i].nq);
     for (long long e = 0; e < s->inst[i].ne; e += per) items.push_back(make_int2(i, (int)e));
-    lds = std::max(lds, ens_item_lds(s->inst[i].nq));
+    lds = std::max(lds, ens_item_lds(s->inst[i].nq, (flags & DGK_FLAG_CONST_VELOCITY) != 0));
   }
   s->nitems = (int)items.size();
   s->lds = lds;

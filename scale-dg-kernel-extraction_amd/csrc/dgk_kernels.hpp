@@ -354,142 +354,6 @@ __global__ void __launch_bounds__(StageShape<NQ>::NT)
   }
 }
 
-// Shape of a cooperative element group: NTH threads stage EPB elements.
-template <int NQ, int NTH>
-struct GroupShape {
-  static constexpr int NP = NQ * NQ * NQ;
-  static constexpr int EPB = NP >= NTH ? 1 : NTH / NP;
-  static constexpr int NN = EPB * NP;
-  static constexpr int NPT = (NN + NTH - 1) / NTH;
-  static constexpr int NF = EPB * 6 * NQ * NQ;
-  static constexpr size_t LDS = sizeof(double) * (3 * NN + NF + NQ * NQ);
-};
-
-// One fused stage over the element group g (elements g*EPB ..) of a
-// compressed-connectivity mesh, NTH threads cooperating (tid = 0..NTH-1):
-// the thread-per-node algorithm of stage_kernel without its cross-group
-// prefetch.  FACT: lift by the 1D vectors lfac = [l_-, l_+]; CV: constant
-// velocity P.cvel.  fsg and cvg are device copies of P.fs and P.cvel: the
-// face loop indexes them at run time, which on P itself (a local copy in the
-// caller) would move P to scratch memory.  Every thread of the workgroup
-// must call it.
-template <int NQ, int NTH, int ORDER, bool FMA, bool FACT, bool CV>
-__device__ __forceinline__ void node_group(const StageParams &P, const double *__restrict__ lfac,
-                                           const double *__restrict__ fsg,
-                                           const double *__restrict__ cvg, long long g, int tid,
-                                           double *smem) {
-  using S = GroupShape<NQ, NTH>;
-  constexpr int NP = S::NP, NFP = NQ * NQ, NFT = 6 * NFP;
-  constexpr int EPB = S::EPB, NN = S::NN, NPT = S::NPT, NF = S::NF;
-  constexpr int NFPT = (NF + NTH - 1) / NTH;
-  double *sfx = smem, *sfy = smem + NN, *sfz = smem + 2 * NN;
-  double *sfb = smem + 3 * NN, *sD = smem + 3 * NN + NF;
-  for (int t = tid; t < NQ * NQ; t += NTH) sD[t] = P.D[t];
-  const bool need_q0 = P.mode == kModeStageN;
-  // HOLD: q and q0 stay in registers between the phases; with more nodes per
-  // thread they are read again in phase C (from L2), which keeps the loop
-  // rolled and the register count low.
-  constexpr bool HOLD = NPT <= 2;
-  constexpr int NH = HOLD ? NPT : 1;  // also the unroll count of phase C
-  double cq[NH], cq0[NH];
-#pragma unroll
-  for (int r = 0; r < NPT; ++r) {
-    const int m = tid + r * NTH;
-    if (HOLD) {
-      cq[HOLD ? r : 0] = 0.0;
-      cq0[HOLD ? r : 0] = 0.0;
-    }
-    if (m < NN && g * EPB + m / NP < P.Ne) {
-      const long long idx = g * NN + m;
-      const double q = P.qin[idx];
-      const double u = CV ? P.cvel[0] : P.u[idx];
-      const double v = CV ? P.cvel[1] : P.v[idx];
-      const double w = CV ? P.cvel[2] : P.w[idx];
-      if (HOLD) {
-        cq[HOLD ? r : 0] = q;
-        cq0[HOLD ? r : 0] = need_q0 ? P.q0[idx] : 0.0;
-      }
-      sfx[m] = q * u;  // mod_advect3d_eq.f90:181-183
-      sfy[m] = q * v;
-      sfz[m] = q * w;
-    }
-  }
-#pragma unroll
-  for (int r = 0; r < NFPT; ++r) {
-    const int t = tid + r * NTH;
-    if (t < NF) {
-      const int el = t / NFT, tt = t - el * NFT;
-      const long long e = g * EPB + el;
-      if (e < P.Ne) {
-        const int f = tt / NFP, fp = tt - f * NFP;
-        const int a = fp % NQ, b = fp / NQ;
-        const long long iM = e * NP + face_node<NQ>(f, a, b);
-        const long long iP = (long long)P.nbr[e * 6 + f] * NP + face_node<NQ>(face_opp(f), a, b);
-        double vm, vp;
-        if constexpr (CV) {
-          vm = vp = cvg[face_axis(f)];
-        } else {
-          const int ax = face_axis(f);
-          const double *vel = ax == 0 ? P.u : (ax == 1 ? P.v : P.w);
-          vm = vel[iM];
-          vp = vel[iP];
-        }
-        const bool plus = face_plus(f);
-        sfb[t] = upwind_flux(P.qin[iM], P.qin[iP], plus ? vm : -vm, plus ? vp : -vp, fsg[f]);
-      }
-    }
-  }
-  __syncthreads();
-#pragma unroll NH
-  for (int r = 0; r < NPT; ++r) {
-    const int m = tid + r * NTH;
-    const long long e = g * EPB + m / NP;
-    if (m < NN && e < P.Ne) {
-      const int el = m / NP, n = m - el * NP;
-      const int i = n % NQ, j = (n / NQ) % NQ, k = n / NFP;
-      const double *bx = sfx + el * NP, *by = sfy + el * NP, *bz = sfz + el * NP;
-      const double dx = contract<NQ, ORDER, FMA>([&](int l) { return sD[i + l * NQ]; },
-                                                 [&](int l) { return bx[l + j * NQ + k * NFP]; });
-      const double dy = contract<NQ, ORDER, FMA>([&](int l) { return by[i + l * NQ + k * NFP]; },
-                                                 [&](int l) { return sD[j + l * NQ]; });
-      const double dz = contract<NQ, ORDER, FMA>([&](int l) { return bz[i + j * NQ + l * NFP]; },
-                                                 [&](int l) { return sD[k + l * NQ]; });
-      const double *fb = sfb + el * NFT;
-      double L[6];
-      if constexpr (FACT) {
-        L[0] = lfac[j]; L[1] = lfac[NQ + i]; L[2] = lfac[NQ + j];
-        L[3] = lfac[i]; L[4] = lfac[k]; L[5] = lfac[NQ + k];
-      } else {
-#pragma unroll
-        for (int f = 0; f < 6; ++f) L[f] = P.Lift[n + f * NP];
-      }
-      double lift = L[0] * fb[i + k * NQ];
-      lift = madd<FMA>(L[1], fb[NFP + j + k * NQ], lift);
-      lift = madd<FMA>(L[2], fb[2 * NFP + i + k * NQ], lift);
-      lift = madd<FMA>(L[3], fb[3 * NFP + j + k * NQ], lift);
-      lift = madd<FMA>(L[4], fb[4 * NFP + i + j * NQ], lift);
-      lift = madd<FMA>(L[5], fb[5 * NFP + i + j * NQ], lift);
-      double s = P.es[0] * dx;
-      s = madd<FMA>(P.es[1], dy, s);
-      s = madd<FMA>(P.es[2], dz, s);
-      const double dq = -(s + lift);
-      const long long idx = g * NN + m;
-      if (P.mode == kModeTend) {
-        P.dqdt[idx] = dq;
-      } else {
-        const double q = HOLD ? cq[HOLD ? r : 0] : P.qin[idx];
-        const double q0v = HOLD ? cq0[HOLD ? r : 0] : (need_q0 ? P.q0[idx] : 0.0);
-        const double q0s = P.mode == kModeStage1 ? q : q0v;
-        if (P.mode == kModeStage1) P.q0[idx] = q;
-        double qn;
-        if constexpr (FMA) qn = __builtin_fma(P.a, q0s, P.b * __builtin_fma(P.dt, dq, q));
-        else qn = P.a * q0s + P.b * (q + P.dt * dq);
-        P.qout[idx] = qn;
-      }
-    }
-  }
-}
-
 // Batched tensorprod_divlike_dirXYZ: one element per workgroup.
 template <int NQ, int ORDER>
 __global__ void __launch_bounds__(256)

@@ -1,0 +1,39 @@
+// Launch descriptors shared by the solver's translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "dgk_kernels.hpp"
+
+namespace dgk {
+
+// Device copies of the 1D operator in row-major order (D(k,l) at
+// [k*Nq + l], read a row at a time by the pencil kernel) and of the vectors
+// [l_-, l_+] of the factored lift.
+struct HostOps {
+  const double *Drow;
+  const double *lfac;
+  const double *LiftP;  // pair-packed Lift for the pencil kernel
+};
+
+struct StageKernel;
+using LaunchFn = void (*)(const StageKernel &, const StageParams &, const HostOps &,
+                          const double *Lift, hipStream_t);
+
+struct StageKernel {
+  const void *kernel = nullptr;  // for attributes and occupancy
+  LaunchFn launch = nullptr;
+  int nt = 0;
+  size_t lds = 0;
+  int epb = 1;       // elements per workgroup (node kernel, group pencil) or per
+                     // wavefront (one-wavefront pencil)
+  bool pencil = false;
+  long long grid = 0;
+};
+
+// The multi-wavefront pencil stage (pencil_group_kernel, lean LDS layout)
+// for Nq in {2, 4, 6, 8, 12, 16} with epb elements per workgroup; an empty
+// descriptor if epb does not fit the kernel's thread limit.
+StageKernel pick_pencil_group(int Nq, int order, bool fma, bool fact, bool cv, int epb);
+
+}  // namespace dgk

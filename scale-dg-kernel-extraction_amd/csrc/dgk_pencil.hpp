@@ -155,8 +155,9 @@ __device__ __forceinline__ void sweep(const double *Drow, const double (&f)[NQ],
 //   kPencilLinearOrder  linear element order instead of the tiled order on
 //                       structured meshes (a launch parameter, not a
 //                       template argument)
-//   kPencilWide         Nq 6, 12: the wide workgroup shape (more elements
-//                       per workgroup)
+//   kPencilWide         the wide workgroup shape: Nq 2, 4, 8 the
+//                       four-wavefront lean group kernel instead of one
+//                       wavefront, Nq 6, 12 more elements per workgroup
 constexpr int kPencilNtStore = 1;
 constexpr int kPencilLinearOrder = 2;
 constexpr int kPencilWide = 4;
@@ -216,7 +217,8 @@ __device__ __forceinline__ void pencil_element(const StageParams &P, const doubl
       v[k] = P.v[base + NFP * k];
       w[k] = P.w[base + NFP * k];
     }
-    q0[k] = need_q0 ? P.q0[base + NFP * k] : 0.0;
+    // LEAN reads q0 in the final phase
+    if constexpr (!LEAN) q0[k] = need_q0 ? P.q0[base + NFP * k] : 0.0;
   }
   const int32_t *nb = P.nbr + eb * 6;
   // z faces at (i,j): -z (face 4) sees the neighbour's k = NQ-1, +z (face 5)
@@ -236,16 +238,20 @@ __device__ __forceinline__ void pencil_element(const StageParams &P, const doubl
   const double vP0 = CV ? P.cvel[1] : P.v[a0], vP2 = CV ? P.cvel[1] : P.v[a2];
   const double uP3 = CV ? P.cvel[0] : P.u[a3], uP1 = CV ? P.cvel[0] : P.u[a1];
 
-  // ---- z phase (registers) ---------------------------------------------
+  // ---- z phase (registers; LEAN: in the final phase) ---------------------
+  // LEAN keeps neither q, q0 nor D_z in registers through the x and y
+  // phases: q is read back from LDS and q0 from memory at the end.
   double dz[NQ];
-  {
+  double flux4 = 0.0, flux5 = 0.0;
+  auto z_phase = [&](const double (&qz)[NQ]) {
     double fz[NQ];
 #pragma unroll
-    for (int k = 0; k < NQ; ++k) fz[k] = q[k] * w[k];
+    for (int k = 0; k < NQ; ++k) fz[k] = qz[k] * w[k];
     sweep<NQ, ORDER, FMA, false>(Drow, fz, dz);
-  }
-  const double flux4 = upwind_flux(q[0], qP4, -w[0], -wP4, P.fs[4]);
-  const double flux5 = upwind_flux(q[NQ - 1], qP5, w[NQ - 1], wP5, P.fs[5]);
+    flux4 = upwind_flux(qz[0], qP4, -w[0], -wP4, P.fs[4]);
+    flux5 = upwind_flux(qz[NQ - 1], qP5, w[NQ - 1], wP5, P.fs[5]);
+  };
+  if constexpr (!LEAN) z_phase(q);
   if (act) {
 #pragma unroll
     for (int k = 0; k < NQ; ++k) {
@@ -354,6 +360,13 @@ __device__ __forceinline__ void pencil_element(const StageParams &P, const doubl
   // ---- final z phase: lift, combine, RK ----------------------------------
   if (!valid) return;
   const int i = la, j = lb;
+  if constexpr (LEAN) {
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) q0[k] = need_q0 ? P.q0[base + NFP * k] : 0.0;
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) q[k] = Bq[pidx<NQ>(pl + NFP * k)];
+    z_phase(q);
+  }
 #pragma unroll
   for (int k = 0; k < NQ; ++k) {
     const int s = pidx<NQ>(pl + NFP * k);
