@@ -158,6 +158,24 @@ int dgk_set_variant(dgk_solver *s, int variant);
 /* Times `nstage` consecutive stage launches with HIP events on the solver's
  * stream; returns the mean milliseconds per stage launch. */
 int dgk_time_stages(dgk_solver *s, double dt, int nstage, double *ms_per_stage);
+/* Autotuner: times every tuning variant (workgroup shape, element order,
+ * store flavour; results are the same bits) over nrep stages at this dt,
+ * keeps the fastest and leaves the state as it was.  variant_out and
+ * ms_out (average stage time of the winner) may be NULL. */
+int dgk_autotune(dgk_solver *s, double dt, int nrep, int *variant_out,
+                 double *ms_out);
+
+/* ---- diagnostics (device reductions) ------------------------------------ */
+/* out[8] for the owned state q, in a fixed reduction order:
+ *   0 min q, 1 max q (the main.f90:76-79 line),
+ *   2 mass = sum_e J_e sum_ijk w_i w_j w_k q, 3 the same sum of |q|
+ *     (w: the LGL weights w1D[Nq], J = 1/(Es1 Es2 Es3) = element volume/8),
+ *   4 sum q^2 over owned DOF,
+ *   5 sum (q-ref)^2, 6 sum ref^2, 7 max |q-ref| for a device field
+ *     ref(Np,Ne) (e.g. dgk_field_ptr of another solver), or 0 if ref is NULL.
+ * Relative L2 error = sqrt(out[5]/out[6]). */
+int dgk_diagnostics(dgk_solver *s, const double *w1D, const double *ref,
+                    double *out);
 
 /* ---- stateless drop-in of the mesh-level tendency ----------------------- */
 /* Same arguments and semantics as advect3d_eq_cal_tend

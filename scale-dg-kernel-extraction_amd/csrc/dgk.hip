@@ -73,6 +73,97 @@ __global__ void __launch_bounds__(256)
   }
 }
 
+// Diagnostics of the owned state q (and of q - r for a reference field r),
+// per-block partials then one block:
+//   0 min q   1 max q   2 sum J w q   3 sum J w |q|   4 sum q^2
+//   5 sum (q-r)^2   6 sum r^2   7 max |q-r|
+// w = w_i w_j w_k (LGL weights of node n), J = 1/(Es1 Es2 Es3) = the element
+// volume / 8: the sums 2 and 3 are the mass and the |q|-mass of the field.
+// Either es (compressed: per-axis Escale) or escale (Np,Ne,3) is used.
+constexpr int kDiag = 8;
+
+struct DiagArgs {
+  const double *q, *r;  // r may be null
+  const double *w1;     // LGL weights (Nq)
+  const double *escale; // (Np,Ne,3) or null
+  double es[3];
+  long long n, NpNe;
+  int Nq, Np;
+};
+
+__device__ __forceinline__ void diag_combine(double *a, const double *b) {
+  a[0] = fmin(a[0], b[0]);
+  a[1] = fmax(a[1], b[1]);
+#pragma unroll
+  for (int t = 2; t < 7; ++t) a[t] += b[t];
+  a[7] = fmax(a[7], b[7]);
+}
+
+__device__ __forceinline__ void diag_block_reduce(double (&v)[kDiag], double *out) {
+  __shared__ double sm[kDiag][256];
+#pragma unroll
+  for (int t = 0; t < kDiag; ++t) sm[t][threadIdx.x] = v[t];
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) {
+      double a[kDiag], b[kDiag];
+#pragma unroll
+      for (int t = 0; t < kDiag; ++t) {
+        a[t] = sm[t][threadIdx.x];
+        b[t] = sm[t][threadIdx.x + h];
+      }
+      diag_combine(a, b);
+#pragma unroll
+      for (int t = 0; t < kDiag; ++t) sm[t][threadIdx.x] = a[t];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int t = 0; t < kDiag; ++t) out[t] = sm[t][0];
+}
+
+__device__ __forceinline__ void diag_init(double (&v)[kDiag]) {
+  v[0] = __builtin_inf();
+  v[1] = -__builtin_inf();
+#pragma unroll
+  for (int t = 2; t < kDiag; ++t) v[t] = 0.0;
+}
+
+__global__ void __launch_bounds__(256) diag_partial_kernel(const DiagArgs A, double *part) {
+  double v[kDiag];
+  diag_init(v);
+  const double J0 = 1.0 / (A.es[0] * A.es[1] * A.es[2]);
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < A.n;
+       t += (long long)gridDim.x * blockDim.x) {
+    const double x = A.q[t];
+    const int nd = (int)(t % A.Np);
+    const int i = nd % A.Nq, j = (nd / A.Nq) % A.Nq, k = nd / (A.Nq * A.Nq);
+    const double J =
+        A.escale ? 1.0 / (A.escale[t] * A.escale[t + A.NpNe] * A.escale[t + 2 * A.NpNe]) : J0;
+    const double wj = J * (A.w1[i] * A.w1[j] * A.w1[k]);
+    v[0] = fmin(v[0], x);
+    v[1] = fmax(v[1], x);
+    v[2] += wj * x;
+    v[3] += wj * fabs(x);
+    v[4] += x * x;
+    if (A.r) {
+      const double r = A.r[t], d = x - r;
+      v[5] += d * d;
+      v[6] += r * r;
+      v[7] = fmax(v[7], fabs(d));
+    }
+  }
+  diag_block_reduce(v, part + kDiag * blockIdx.x);
+}
+
+__global__ void __launch_bounds__(256) diag_final_kernel(const double *part, int nb, double *out) {
+  double v[kDiag];
+  diag_init(v);
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) diag_combine(v, part + kDiag * b);
+  diag_block_reduce(v, out);
+}
+
 }  // namespace dgk
 
 #define HIPOK(expr)                                                            \
@@ -124,15 +215,16 @@ StageKernel make_pencil() {
   return k;
 }
 
-// Elements per workgroup of the multi-wavefront pencil: Nq 2, 4, 8 (wide
-// variant only) four wavefronts; Nq 6 narrow 7 elements (four wavefronts) or
-// wide 16 (nine); Nq 12 narrow 1 (three) or wide 2 (five); Nq 16 one.
+// Elements per workgroup of the group pencil kernel: Nq 2, 4, 8 (variant
+// bit kPencilWide only) one wavefront, as pencil_kernel but in the lean
+// layout; Nq 6 narrow 7 elements (four wavefronts) or wide 16 (nine); Nq 12
+// narrow 1 (three) or wide 2 (five); Nq 16 one.
 int group_epb(int Nq, bool wide) {
   switch (Nq) {
   case 6: return wide ? 16 : 7;
   case 12: return wide ? 2 : 1;
   case 16: return 1;
-  default: return 256 / (Nq * Nq);
+  default: return 64 / (Nq * Nq);
   }
 }
 
@@ -165,8 +257,8 @@ bool pencil_nq(int Nq) {
   return Nq == 2 || Nq == 4 || Nq == 6 || Nq == 8 || Nq == 12 || Nq == 16;
 }
 
-// Nq 2, 4, 8: the one-wavefront pencil_kernel, or with `wide` the
-// four-wavefront lean group kernel; Nq 6, 12, 16: the group kernel.
+// Nq 2, 4, 8: pencil_kernel, or with `wide` the group kernel (one
+// wavefront, lean layout); Nq 6, 12, 16: the group kernel.
 StageKernel pick_pencil(int Nq, int order, bool fma, bool fact, bool nt, bool cv, bool wide) {
   if (Nq == 6 || Nq == 12 || Nq == 16 || wide)
     return pick_pencil_group(Nq, order, fma, fact, cv, group_epb(Nq, wide));
@@ -253,6 +345,7 @@ struct dgk_solver {
   double *nrm = nullptr, *fscale = nullptr, *escale = nullptr;
   double fs[6] = {0, 0, 0, 0, 0, 0}, es[3] = {0, 0, 0};
   double *part = nullptr, *mm = nullptr;
+  double *diag = nullptr;  // diagnostics: partials (8 x 1024), weights (Nq), result (8)
   int mm_cap = 0;
   int cur = 0;
   hipStream_t stream = nullptr;
@@ -550,7 +643,7 @@ int dgk_destroy(dgk_solver *s) {
   drop_graphs(s);
   void *bufs[] = {s->q[0], s->q[1], s->q0, s->u, s->v, s->w, s->dqdt, s->D, s->Lift, s->lfac, s->Drow, s->LiftP,
                   s->nbr, s->vmapM, s->vmapP, s->halo, s->nrm, s->fscale, s->escale,
-                  s->part, s->mm};
+                  s->part, s->mm, s->diag};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -854,6 +947,81 @@ int dgk_set_variant(dgk_solver *s, int variant) {
   s->variant = variant;
   drop_graphs(s);
   return configure_launch(s);
+}
+
+int dgk_diagnostics(dgk_solver *s, const double *w1D, const double *ref, double *out) {
+  if (!s || !w1D || !out) return fail(DGK_ERR_ARG, "bad argument");
+  if (!s->fields_set) return fail(DGK_ERR_STATE, "dgk_set_fields has not been called");
+  constexpr int kMaxBlocks = 1024;
+  if (!s->diag) {
+    int rc = dalloc(&s->diag, (size_t)kDiag * kMaxBlocks + 16 + kDiag);
+    if (rc) return rc;
+  }
+  double *part = s->diag, *w = s->diag + kDiag * kMaxBlocks, *res = w + 16;
+  HIPOK(hipMemcpyAsync(w, w1D, s->Nq * sizeof(double), hipMemcpyHostToDevice, s->stream));
+  DiagArgs A{};
+  A.q = s->q[s->cur];
+  A.r = ref;
+  A.w1 = w;
+  A.escale = s->nbr ? nullptr : s->escale;
+  for (int d = 0; d < 3; ++d) A.es[d] = s->nbr ? s->es[d] : 1.0;
+  A.n = (long long)s->Np * s->Ne;
+  A.NpNe = A.n;
+  A.Nq = s->Nq;
+  A.Np = s->Np;
+  const int nb = (int)std::min<long long>(kMaxBlocks, (A.n + 255) / 256);
+  hipLaunchKernelGGL(diag_partial_kernel, dim3(nb), dim3(256), 0, s->stream, A, part);
+  HIPOK(hipGetLastError());
+  hipLaunchKernelGGL(diag_final_kernel, dim3(1), dim3(256), 0, s->stream, part, nb, res);
+  HIPOK(hipGetLastError());
+  HIPOK(hipMemcpyAsync(out, res, kDiag * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  HIPOK(hipStreamSynchronize(s->stream));
+  return DGK_OK;
+}
+
+int dgk_autotune(dgk_solver *s, double dt, int nrep, int *variant_out, double *ms_out) {
+  if (!s || nrep < 1) return fail(DGK_ERR_ARG, "bad argument");
+  if (!s->fields_set) return fail(DGK_ERR_STATE, "dgk_set_fields has not been called");
+  const int nvar = pencil_variants(s);
+  if (nvar <= 1) {
+    if (variant_out) *variant_out = 0;
+    if (ms_out) return dgk_time_stages(s, dt, nrep, ms_out);
+    return DGK_OK;
+  }
+  // the trial stages advance the state: keep q and q0 aside and restore them
+  HIPOK(hipStreamSynchronize(s->stream));
+  const size_t bytes = (size_t)s->Np * (size_t)s->NeA * sizeof(double);
+  double *save = nullptr;
+  int rc = dalloc(&save, 2 * (size_t)s->Np * (size_t)s->NeA);
+  if (rc) return rc;
+  const int cur0 = s->cur;
+  int best = -1;
+  double best_ms = 0.0;
+  if (hipMemcpy(save, s->q[cur0], bytes, hipMemcpyDeviceToDevice) != hipSuccess ||
+      hipMemcpy(save + bytes / sizeof(double), s->q0, bytes, hipMemcpyDeviceToDevice) !=
+          hipSuccess)
+    rc = fail(DGK_ERR_HIP, "autotune: state copy failed");
+  for (int v = 0; v < nvar && !rc; ++v) {
+    double ms = 0.0;
+    if ((rc = dgk_set_variant(s, v))) break;
+    if ((rc = dgk_time_stages(s, dt, 3, &ms))) break;  // warm-up
+    if ((rc = dgk_time_stages(s, dt, nrep, &ms))) break;
+    if (best < 0 || ms < best_ms) {
+      best = v;
+      best_ms = ms;
+    }
+  }
+  s->cur = cur0;
+  if (!rc && (hipMemcpy(s->q[cur0], save, bytes, hipMemcpyDeviceToDevice) != hipSuccess ||
+              hipMemcpy(s->q0, save + bytes / sizeof(double), bytes, hipMemcpyDeviceToDevice) !=
+                  hipSuccess))
+    rc = fail(DGK_ERR_HIP, "autotune: state restore failed");
+  (void)hipFree(save);
+  if (rc) return rc;
+  if ((rc = dgk_set_variant(s, best))) return rc;
+  if (variant_out) *variant_out = best;
+  if (ms_out) *ms_out = best_ms;
+  return DGK_OK;
 }
 
 int dgk_time_stages(dgk_solver *s, double dt, int nstage, double *ms_per_stage) {

@@ -155,9 +155,8 @@ __device__ __forceinline__ void sweep(const double *Drow, const double (&f)[NQ],
 //   kPencilLinearOrder  linear element order instead of the tiled order on
 //                       structured meshes (a launch parameter, not a
 //                       template argument)
-//   kPencilWide         the wide workgroup shape: Nq 2, 4, 8 the
-//                       four-wavefront lean group kernel instead of one
-//                       wavefront, Nq 6, 12 more elements per workgroup
+//   kPencilWide         Nq 2, 4, 8: the group kernel's lean layout (one
+//                       wavefront); Nq 6, 12: more elements per workgroup
 constexpr int kPencilNtStore = 1;
 constexpr int kPencilLinearOrder = 2;
 constexpr int kPencilWide = 4;

@@ -36,7 +36,7 @@ EXPORTS = (
     "dgk_lift_hexahedral_batched", "dgk_ensemble_create", "dgk_ensemble_destroy",
     "dgk_ensemble_set_fields", "dgk_ensemble_get_field", "dgk_ensemble_set_dt",
     "dgk_ensemble_step", "dgk_ensemble_run", "dgk_ensemble_time_stages",
-    "dgk_ensemble_tune", "dgk_ensemble_query",
+    "dgk_ensemble_tune", "dgk_ensemble_query", "dgk_autotune", "dgk_diagnostics",
 )
 
 
@@ -85,6 +85,8 @@ def lib():
         "dgk_advect3d_eq_cal_tend": ([vp] * 13 + [i] * 6, i),
         "dgk_divlike_dirXYZ_batched": ([i, i, i] + [vp] * 9, i),
         "dgk_lift_hexahedral_batched": ([i, i, vp, vp, vp, vp], i),
+        "dgk_autotune": ([vp, d, i, vp, vp], i),
+        "dgk_diagnostics": ([vp, vp, vp, vp], i),
         "dgk_ensemble_create": ([vp, i] + [vp] * 5 + [C.c_char_p, i, i], i),
         "dgk_ensemble_destroy": ([vp], i),
         "dgk_ensemble_set_fields": ([vp, i] + [vp] * 4, i),
@@ -280,6 +282,35 @@ class Solver:
         ms = C.c_double(0.0)
         _check(lib().dgk_time_stages(self._h, float(dt), nstage, C.byref(ms)))
         return ms.value
+
+    def autotune(self, dt, nrep=10):
+        """Selects the fastest tuning variant; returns (variant, ms per stage)."""
+        v, ms = C.c_int(0), C.c_double(0.0)
+        _check(lib().dgk_autotune(self._h, float(dt), nrep, C.byref(v), C.byref(ms)))
+        return v.value, ms.value
+
+    def diagnostics(self, x1D, ref=None):
+        """Device reductions of the state: min, max, LGL mass, |q|-mass, sum
+        q^2 and, against ref (another Solver or a device address of a (Np,Ne)
+        field), rel_l2 and max_err."""
+        w = lgl_weights(x1D)
+        out = np.zeros(8)
+        rp = ref.field_ptr() if isinstance(ref, Solver) else ref
+        _check(lib().dgk_diagnostics(self._h, _p(w), rp, _p(out)))
+        d = dict(min=out[0], max=out[1], mass=out[2], abs_mass=out[3], sum_sq=out[4])
+        if rp:
+            d.update(rel_l2=float(np.sqrt(out[5] / out[6])) if out[6] > 0 else float("nan"),
+                     max_err=out[7], err_sq=out[5], ref_sq=out[6])
+        return d
+
+
+def lgl_weights(x1D):
+    """Quadrature weights of the Legendre-Gauss-Lobatto nodes x1D:
+    w_i = 2 / (p (p+1) P_p(x_i)^2)."""
+    x1D = np.asarray(x1D, dtype=np.float64)
+    p = len(x1D) - 1
+    P = np.polynomial.legendre.Legendre.basis(p)(x1D)
+    return np.ascontiguousarray(2.0 / (p * (p + 1) * P * P))
 
 
 def advect3d_eq_cal_tend(q, u, v, w, D1D, D1D_tr, Lift, VMapM, VMapP, normal_fn,
