@@ -205,6 +205,32 @@ int dgk_divlike_dirXYZ_batched(int Nq, int nelem, int type_id,
 int dgk_lift_hexahedral_batched(int Nq, int nelem, const double *Lift,
                                 const double *in, double *out, void *stream);
 
+/* The rest of the OPT1 operator family (mod_dg_optr_kernel_opt1.F90.erb),
+ * over nelem element blocks (device arrays, (Np,nelem) each):
+ * element_operation_kernel_matvec_dir{X,Y,Z}_P<p> (:51-129): dir 0
+ * out(i,jk) = Mat(i,l) in(l,jk), dir 1 out(i,j,k) = in(i,l,k) Mat(l,j), dir 2
+ * out(i,j,k) = in(i,j,l) Mat(l,k) (Mat = Mat_x, or the transposed Mat_y,
+ * Mat_z as the reference's callers pass them); left-to-right sums. */
+int dgk_matvec_dir_batched(int Nq, int nelem, int dir, const double *Mat,
+                           const double *in, double *out, void *stream);
+/* element_operation_kernel_matvec_gradlike_dirXYZ_P<p> (:166-219) in its
+ * stated form: the x, y, z derivatives of one field (in(Np,nelem)).  The
+ * reference's z loop begins with in(i,1,k)*Mat_tr(1,j) (:208); here it is
+ * in(i,j,1)*Mat_tr(1,k), so out_z differs from the reference's. */
+int dgk_gradlike_dirXYZ_batched(int Nq, int nelem, const double *Mat,
+                                const double *Mat_tr, const double *in,
+                                double *out_x, double *out_y, double *out_z,
+                                void *stream);
+/* element_operation_kernel_matvec_divlike_dirXYZ_var5_P<p> (:347-465):
+ * divlike_dirXYZ in OPT1's order on 5 fields per element (in/out
+ * (Np,5,nelem)).  The reference's y loop indexes its output with
+ * j + (kv-1)*Nq*5 (:400); here with the intended j + (kv-1)*Nq. */
+int dgk_divlike_dirXYZ_var5_batched(int Nq, int nelem, const double *Mat,
+                                    const double *Mat_tr, const double *in_x,
+                                    const double *in_y, const double *in_z,
+                                    double *out_x, double *out_y,
+                                    double *out_z, void *stream);
+
 /* ---- batched ensemble ---------------------------------------------------- */
 /* n independent runs of the reference program (main.f90:51-80), each on its
  * own periodic NeX x NeY x NeZ mesh of order p with its own velocity field,

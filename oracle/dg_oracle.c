@@ -308,6 +308,65 @@ void orc_divlike(int Nq, int typeid, const double *Mat, const double *Mat_tr,
       }
 }
 
+/* Left-to-right sum a(1)*b(1) + a(2)*b(2) + ... + a(Nq)*b(Nq) with no split
+ * at any p: the directional kernels dirX/dirY/dirZ and gradlike_dirXYZ of the
+ * OPT1 module (mod_dg_optr_kernel_opt1.F90.erb:63-66, 92-95, 121-124,
+ * 184-206). */
+static double sum_seq(const double *a, long sa, const double *b, long sb, int Nq) {
+  double s = a[0] * b[0];
+  for (int l = 1; l < Nq; ++l) s = s + a[l * sa] * b[l * sb];
+  return s;
+}
+
+/* element_operation_kernel_matvec_dir{X,Y,Z}_P<p> for one element
+ * (mod_dg_optr_kernel_opt1.F90.erb:51-129): dir 0 out(i,jk) = Mat(i,l)
+ * in(l,jk); dir 1 out(i,j,k) = in(i,l,k) Mat(l,j); dir 2 out(i,j,k) =
+ * in(i,j,l) Mat(l,k).  Mat is Mat_x for x and the transposed Mat_y / Mat_z
+ * for y and z, as the reference's callers pass them. */
+void orc_matvec_dir(int Nq, int dir, const double *Mat, const double *in, double *out) {
+  int Nq2 = Nq * Nq;
+  for (int k = 0; k < Nq; ++k)
+    for (int j = 0; j < Nq; ++j)
+      for (int i = 0; i < Nq; ++i) {
+        long n = i + j * Nq + (long)k * Nq2;
+        if (dir == 0)
+          out[n] = sum_seq(&Mat[i], Nq, &in[j * Nq + (long)k * Nq2], 1, Nq);
+        else if (dir == 1)
+          out[n] = sum_seq(&in[i + (long)k * Nq2], Nq, &Mat[j * Nq], 1, Nq);
+        else
+          out[n] = sum_seq(&in[i + j * Nq], Nq2, &Mat[k * Nq], 1, Nq);
+      }
+}
+
+/* element_operation_kernel_matvec_gradlike_dirXYZ_P<p>
+ * (mod_dg_optr_kernel_opt1.F90.erb:166-219) as its comment states it: the
+ * x, y and z derivatives of one field, out_x = Mat along x, out_y and out_z =
+ * Mat_tr along y and z, left-to-right sums.  The reference's z loop
+ * (:208-213) starts its sum with in(i,1,k)*Mat_tr(1,j) instead of
+ * in(i,j,1)*Mat_tr(1,k); that slip is not reproduced here (the tests show
+ * the difference on the reference's own output). */
+void orc_gradlike(int Nq, const double *Mat, const double *Mat_tr, const double *in,
+                  double *out_x, double *out_y, double *out_z) {
+  orc_matvec_dir(Nq, 0, Mat, in, out_x);
+  orc_matvec_dir(Nq, 1, Mat_tr, in, out_y);
+  orc_matvec_dir(Nq, 2, Mat_tr, in, out_z);
+}
+
+/* element_operation_kernel_matvec_divlike_dirXYZ_var5_P<p>
+ * (mod_dg_optr_kernel_opt1.F90.erb:347-465): divlike_dirXYZ in OPT1's order
+ * on five consecutive fields v = 1..5 of one element (in/out (Np,5)).  The
+ * reference's y loop writes out_y at jkv = j + (kv-1)*Nq*5 (:400), which
+ * leaves most of out_y unwritten and runs past its end for kv > Nq; the
+ * intended index j + (kv-1)*Nq is used here. */
+void orc_divlike_var5(int Nq, const double *Mat, const double *Mat_tr, const double *in_x,
+                      const double *in_y, const double *in_z, double *out_x, double *out_y,
+                      double *out_z) {
+  long Np = (long)Nq * Nq * Nq;
+  for (int v = 0; v < 5; ++v)
+    orc_divlike(Nq, ORC_OPT1, Mat, Mat_tr, in_x + v * Np, in_y + v * Np, in_z + v * Np,
+                out_x + v * Np, out_y + v * Np, out_z + v * Np);
+}
+
 /* tensorprod_Lift_hexahedral for one element (mod_dg_optr_kernel.f90:262-288,
  * mod_dg_optr_kernel_opt1.F90.erb:134-157; both use the same order) */
 void orc_lift(int Nq, const double *Lift, const double *in, double *out) {

@@ -53,6 +53,9 @@ def lib():
     L.orc_elembnd_flux.argtypes = [vp] * 9 + [i, i, i]
     L.orc_divlike.argtypes = [i, i] + [vp] * 8
     L.orc_lift.argtypes = [i, vp, vp, vp]
+    L.orc_matvec_dir.argtypes = [i, i, vp, vp, vp]
+    L.orc_gradlike.argtypes = [i] + [vp] * 6
+    L.orc_divlike_var5.argtypes = [i] + [vp] * 8
     L.orc_cal_dqdt.argtypes = [vp] * 10 + [i, i, i]
     L.orc_cal_tend.argtypes = [vp] * 13 + [i, i, i, vp]
     L.orc_run.argtypes = [vp] * 13 + [i, i, i, i, i, d, i, vp]
@@ -159,6 +162,28 @@ def divlike(Nq, typeid, D, Dtr, ix, iy, iz):
     return ox, oy, oz
 
 
+def matvec_dir(Nq, direction, Mat, a):
+    """dirX/dirY/dirZ (direction 0/1/2) of the OPT1 module on one block."""
+    out = np.zeros(Nq ** 3)
+    lib().orc_matvec_dir(Nq, direction, _p(Mat), _p(a), _p(out))
+    return out
+
+
+def gradlike(Nq, D, Dtr, a):
+    """gradlike_dirXYZ (its intended form) on one block."""
+    ox, oy, oz = (np.zeros(Nq ** 3) for _ in range(3))
+    lib().orc_gradlike(Nq, _p(D), _p(Dtr), _p(a), _p(ox), _p(oy), _p(oz))
+    return ox, oy, oz
+
+
+def divlike_var5(Nq, D, Dtr, ix, iy, iz):
+    """divlike_dirXYZ_var5 (its intended form) on one element: (Np*5,) each."""
+    ox, oy, oz = (np.zeros(5 * Nq ** 3) for _ in range(3))
+    lib().orc_divlike_var5(Nq, _p(D), _p(Dtr), _p(ix), _p(iy), _p(iz),
+                           _p(ox), _p(oy), _p(oz))
+    return ox, oy, oz
+
+
 def lift(Nq, Lift, fb):
     """tensorprod_Lift_hexahedral on one element block."""
     out = np.zeros(Nq ** 3)
@@ -244,6 +269,20 @@ def ref_optr(p, tname, ix, iy, iz, fb):
     _ref(["optr", p, tname, nelem, inf, outf], d)
     o = np.fromfile(outf, dtype=np.float64).reshape((ix.shape[0], nelem, 4), order="F")
     return tuple(np.asfortranarray(o[:, :, k]) for k in range(4))
+
+
+def ref_optr2(p, a):
+    """Reference dirX, dirY, dirZ and gradlike_dirXYZ (OPT1 module) over the
+    blocks a(Np,nelem), with D1D for x and D1D_tr for y and z; returns
+    (dx, dy, dz, gx, gy, gz) each (Np,nelem)."""
+    d = _workdir()
+    nelem = a.shape[1]
+    inf, outf = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
+    with open(inf, "wb") as f:
+        f.write(np.asfortranarray(a, dtype=np.float64).tobytes(order="F"))
+    _ref(["optr2", p, nelem, inf, outf], d)
+    o = np.fromfile(outf, dtype=np.float64).reshape((a.shape[0], nelem, 6), order="F")
+    return tuple(np.asfortranarray(o[:, :, k]) for k in range(6))
 
 
 def write_conf(path, NeX, NeY, NeZ, p, dt, nstep, opt="OPT1", output_interval=None):

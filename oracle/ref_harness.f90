@@ -12,6 +12,7 @@
 !! Usage (all arrays are raw little-endian float64 / int32, Fortran order):
 !!   ref_harness mesh <NeX> <NeY> <NeZ> <p> <outfile>
 !!   ref_harness optr <p> <GENERAL|OPT1> <nelem> <infile> <outfile>
+!!   ref_harness optr2 <p> <nelem> <infile> <outfile>
 !!   ref_harness tend <conf> <icfile> <outfile>
 !!   ref_harness run  <conf> <icfile> <outfile>
 !! The working directory must hold operator_data/p<p>.dat (mod_mesh.f90:253).
@@ -24,6 +25,7 @@ program ref_harness
   use mod_dg_optr_kernel, only: dg_optr_kernel_setup, &
        tensorprod_divlike_dirXYZ, tensorprod_Lift_hexahedral
   use mod_advect3d_eq, only: advect3d_eq_cal_tend
+  use mod_dg_optr_kernel_opt1
   use omp_lib, only: omp_get_wtime, omp_get_max_threads
   implicit none
 
@@ -46,6 +48,12 @@ program ref_harness
     call get_command_argument(5, a4)
     call get_command_argument(6, a5)
     call do_optr(p, trim(a2), nelem, trim(a4), trim(a5))
+  case ("optr2")
+    call get_command_argument(2, a1); read(a1,*) p
+    call get_command_argument(3, a3); read(a3,*) nelem
+    call get_command_argument(4, a4)
+    call get_command_argument(5, a5)
+    call do_optr2(p, nelem, trim(a4), trim(a5))
   case ("tend", "run")
     call get_command_argument(2, a1)
     call get_command_argument(3, a2)
@@ -103,6 +111,119 @@ contains
     write(fid) ox, oy, oz, ol
     close(fid)
   end subroutine do_optr
+
+  !> optr2: the OPT1 module's directional kernels dirX/dirY/dirZ
+  !! (mod_dg_optr_kernel_opt1.F90.erb:51-129) and gradlike_dirXYZ (:166-219)
+  !! over nelem element blocks a(Np,nelem), with D1D (x) and D1D_tr (y, z).
+  subroutine do_optr2(p, nelem, infile, outfile)
+    integer, intent(in) :: p, nelem
+    character(len=*), intent(in) :: infile, outfile
+    real(RP), allocatable :: a(:,:), ox(:,:), oy(:,:), oz(:,:), gx(:,:), gy(:,:), gz(:,:)
+    integer :: fid, ke
+    call mesh_setup(1, 1, 1, p, 1.0_RP, 1.0_RP, 1.0_RP)
+    allocate(a(Np,nelem), ox(Np,nelem), oy(Np,nelem), oz(Np,nelem))
+    allocate(gx(Np,nelem), gy(Np,nelem), gz(Np,nelem))
+    open(newunit=fid, file=infile, access='stream', form='unformatted', status='old')
+    read(fid) a
+    close(fid)
+    do ke = 1, nelem
+      select case (p)
+      case (1)
+        call element_operation_kernel_matvec_dirX_P1(D1D, a(:,ke), ox(:,ke))
+        call element_operation_kernel_matvec_dirY_P1(D1D_tr, a(:,ke), oy(:,ke))
+        call element_operation_kernel_matvec_dirZ_P1(D1D_tr, a(:,ke), oz(:,ke))
+        call element_operation_kernel_matvec_gradlike_dirXYZ_P1(D1D, D1D_tr, a(:,ke), a(:,ke), &
+             gx(:,ke), gy(:,ke), gz(:,ke))
+      case (2)
+        call element_operation_kernel_matvec_dirX_P2(D1D, a(:,ke), ox(:,ke))
+        call element_operation_kernel_matvec_dirY_P2(D1D_tr, a(:,ke), oy(:,ke))
+        call element_operation_kernel_matvec_dirZ_P2(D1D_tr, a(:,ke), oz(:,ke))
+        call element_operation_kernel_matvec_gradlike_dirXYZ_P2(D1D, D1D_tr, a(:,ke), a(:,ke), &
+             gx(:,ke), gy(:,ke), gz(:,ke))
+      case (3)
+        call element_operation_kernel_matvec_dirX_P3(D1D, a(:,ke), ox(:,ke))
+        call element_operation_kernel_matvec_dirY_P3(D1D_tr, a(:,ke), oy(:,ke))
+        call element_operation_kernel_matvec_dirZ_P3(D1D_tr, a(:,ke), oz(:,ke))
+        call element_operation_kernel_matvec_gradlike_dirXYZ_P3(D1D, D1D_tr, a(:,ke), a(:,ke), &
+             gx(:,ke), gy(:,ke), gz(:,ke))
+      case (4)
+        call element_operation_kernel_matvec_dirX_P4(D1D, a(:,ke), ox(:,ke))
+        call element_operation_kernel_matvec_dirY_P4(D1D_tr, a(:,ke), oy(:,ke))
+        call element_operation_kernel_matvec_dirZ_P4(D1D_tr, a(:,ke), oz(:,ke))
+        call element_operation_kernel_matvec_gradlike_dirXYZ_P4(D1D, D1D_tr, a(:,ke), a(:,ke), &
+             gx(:,ke), gy(:,ke), gz(:,ke))
+      case (5)
+        call element_operation_kernel_matvec_dirX_P5(D1D, a(:,ke), ox(:,ke))
+        call element_operation_kernel_matvec_dirY_P5(D1D_tr, a(:,ke), oy(:,ke))
+        call element_operation_kernel_matvec_dirZ_P5(D1D_tr, a(:,ke), oz(:,ke))
+        call element_operation_kernel_matvec_gradlike_dirXYZ_P5(D1D, D1D_tr, a(:,ke), a(:,ke), &
+             gx(:,ke), gy(:,ke), gz(:,ke))
+      case (6)
+        call element_operation_kernel_matvec_dirX_P6(D1D, a(:,ke), ox(:,ke))
+        call element_operation_kernel_matvec_dirY_P6(D1D_tr, a(:,ke), oy(:,ke))
+        call element_operation_kernel_matvec_dirZ_P6(D1D_tr, a(:,ke), oz(:,ke))
+        call element_operation_kernel_matvec_gradlike_dirXYZ_P6(D1D, D1D_tr, a(:,ke), a(:,ke), &
+             gx(:,ke), gy(:,ke), gz(:,ke))
+      case (7)
+        call element_operation_kernel_matvec_dirX_P7(D1D, a(:,ke), ox(:,ke))
+        call element_operation_kernel_matvec_dirY_P7(D1D_tr, a(:,ke), oy(:,ke))
+        call element_operation_kernel_matvec_dirZ_P7(D1D_tr, a(:,ke), oz(:,ke))
+        call element_operation_kernel_matvec_gradlike_dirXYZ_P7(D1D, D1D_tr, a(:,ke), a(:,ke), &
+             gx(:,ke), gy(:,ke), gz(:,ke))
+      case (8)
+        call element_operation_kernel_matvec_dirX_P8(D1D, a(:,ke), ox(:,ke))
+        call element_operation_kernel_matvec_dirY_P8(D1D_tr, a(:,ke), oy(:,ke))
+        call element_operation_kernel_matvec_dirZ_P8(D1D_tr, a(:,ke), oz(:,ke))
+        call element_operation_kernel_matvec_gradlike_dirXYZ_P8(D1D, D1D_tr, a(:,ke), a(:,ke), &
+             gx(:,ke), gy(:,ke), gz(:,ke))
+      case (9)
+        call element_operation_kernel_matvec_dirX_P9(D1D, a(:,ke), ox(:,ke))
+        call element_operation_kernel_matvec_dirY_P9(D1D_tr, a(:,ke), oy(:,ke))
+        call element_operation_kernel_matvec_dirZ_P9(D1D_tr, a(:,ke), oz(:,ke))
+        call element_operation_kernel_matvec_gradlike_dirXYZ_P9(D1D, D1D_tr, a(:,ke), a(:,ke), &
+             gx(:,ke), gy(:,ke), gz(:,ke))
+      case (10)
+        call element_operation_kernel_matvec_dirX_P10(D1D, a(:,ke), ox(:,ke))
+        call element_operation_kernel_matvec_dirY_P10(D1D_tr, a(:,ke), oy(:,ke))
+        call element_operation_kernel_matvec_dirZ_P10(D1D_tr, a(:,ke), oz(:,ke))
+        call element_operation_kernel_matvec_gradlike_dirXYZ_P10(D1D, D1D_tr, a(:,ke), a(:,ke), &
+             gx(:,ke), gy(:,ke), gz(:,ke))
+      case (11)
+        call element_operation_kernel_matvec_dirX_P11(D1D, a(:,ke), ox(:,ke))
+        call element_operation_kernel_matvec_dirY_P11(D1D_tr, a(:,ke), oy(:,ke))
+        call element_operation_kernel_matvec_dirZ_P11(D1D_tr, a(:,ke), oz(:,ke))
+        call element_operation_kernel_matvec_gradlike_dirXYZ_P11(D1D, D1D_tr, a(:,ke), a(:,ke), &
+             gx(:,ke), gy(:,ke), gz(:,ke))
+      case (12)
+        call element_operation_kernel_matvec_dirX_P12(D1D, a(:,ke), ox(:,ke))
+        call element_operation_kernel_matvec_dirY_P12(D1D_tr, a(:,ke), oy(:,ke))
+        call element_operation_kernel_matvec_dirZ_P12(D1D_tr, a(:,ke), oz(:,ke))
+        call element_operation_kernel_matvec_gradlike_dirXYZ_P12(D1D, D1D_tr, a(:,ke), a(:,ke), &
+             gx(:,ke), gy(:,ke), gz(:,ke))
+      case (13)
+        call element_operation_kernel_matvec_dirX_P13(D1D, a(:,ke), ox(:,ke))
+        call element_operation_kernel_matvec_dirY_P13(D1D_tr, a(:,ke), oy(:,ke))
+        call element_operation_kernel_matvec_dirZ_P13(D1D_tr, a(:,ke), oz(:,ke))
+        call element_operation_kernel_matvec_gradlike_dirXYZ_P13(D1D, D1D_tr, a(:,ke), a(:,ke), &
+             gx(:,ke), gy(:,ke), gz(:,ke))
+      case (14)
+        call element_operation_kernel_matvec_dirX_P14(D1D, a(:,ke), ox(:,ke))
+        call element_operation_kernel_matvec_dirY_P14(D1D_tr, a(:,ke), oy(:,ke))
+        call element_operation_kernel_matvec_dirZ_P14(D1D_tr, a(:,ke), oz(:,ke))
+        call element_operation_kernel_matvec_gradlike_dirXYZ_P14(D1D, D1D_tr, a(:,ke), a(:,ke), &
+             gx(:,ke), gy(:,ke), gz(:,ke))
+      case (15)
+        call element_operation_kernel_matvec_dirX_P15(D1D, a(:,ke), ox(:,ke))
+        call element_operation_kernel_matvec_dirY_P15(D1D_tr, a(:,ke), oy(:,ke))
+        call element_operation_kernel_matvec_dirZ_P15(D1D_tr, a(:,ke), oz(:,ke))
+        call element_operation_kernel_matvec_gradlike_dirXYZ_P15(D1D, D1D_tr, a(:,ke), a(:,ke), &
+             gx(:,ke), gy(:,ke), gz(:,ke))
+      end select
+    end do
+    open(newunit=fid, file=outfile, access='stream', form='unformatted', status='replace')
+    write(fid) ox, oy, oz, gx, gy, gz
+    close(fid)
+  end subroutine do_optr2
 
   !> tend: one advect3d_eq_cal_tend on the initial state (after halo update).
   !! run : the main.f90 step loop; writes the final owned q, then per-step

@@ -15,6 +15,9 @@ namespace dgk {
 
 constexpr int kGeneral = 1;  // mod_dg_optr_kernel.f90:39
 constexpr int kOpt1 = 2;     // mod_dg_optr_kernel.f90:42
+// Left-to-right sums at every p (no split): the OPT1 module's dirX/dirY/dirZ
+// and gradlike kernels (mod_dg_optr_kernel_opt1.F90.erb:51-129, 166-219).
+constexpr int kSequential = 3;
 constexpr int kNumXcd = 8;   // MI355X: 8 XCDs, workgroups dispatched round-robin
 
 // Stage modes of stage_kernel.
@@ -87,6 +90,7 @@ __device__ __forceinline__ double madd(double x, double y, double s) {
 //   OPT1   : left-to-right for p <= 7; for p > 7 three partial sums over
 //            [0,r), [r,2r), [2r,NQ) with r = round(NQ/3), then t1 + t2 + t3
 //            (mod_dg_optr_kernel_opt1.F90.erb:42-44, 249-342).
+//   SEQUENTIAL: left-to-right at every p.
 template <int NQ, int ORDER, bool FMA, class FA, class FB>
 __device__ __forceinline__ double contract(FA a, FB b) {
   if constexpr (ORDER == kGeneral) {
@@ -94,7 +98,7 @@ __device__ __forceinline__ double contract(FA a, FB b) {
 #pragma unroll
     for (int l = 0; l < NQ; ++l) s = madd<FMA>(a(l), b(l), s);
     return s;
-  } else if constexpr (NQ <= 8) {
+  } else if constexpr (NQ <= 8 || ORDER == kSequential) {
     double s = a(0) * b(0);
 #pragma unroll
     for (int l = 1; l < NQ; ++l) s = madd<FMA>(a(l), b(l), s);

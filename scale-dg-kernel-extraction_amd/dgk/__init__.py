@@ -37,6 +37,7 @@ EXPORTS = (
     "dgk_ensemble_set_fields", "dgk_ensemble_get_field", "dgk_ensemble_set_dt",
     "dgk_ensemble_step", "dgk_ensemble_run", "dgk_ensemble_time_stages",
     "dgk_ensemble_tune", "dgk_ensemble_query", "dgk_autotune", "dgk_diagnostics",
+    "dgk_matvec_dir_batched", "dgk_gradlike_dirXYZ_batched", "dgk_divlike_dirXYZ_var5_batched",
 )
 
 
@@ -86,6 +87,9 @@ def lib():
         "dgk_divlike_dirXYZ_batched": ([i, i, i] + [vp] * 9, i),
         "dgk_lift_hexahedral_batched": ([i, i, vp, vp, vp, vp], i),
         "dgk_autotune": ([vp, d, i, vp, vp], i),
+        "dgk_matvec_dir_batched": ([i, i, i, vp, vp, vp, vp], i),
+        "dgk_gradlike_dirXYZ_batched": ([i, i] + [vp] * 7, i),
+        "dgk_divlike_dirXYZ_var5_batched": ([i, i] + [vp] * 9, i),
         "dgk_diagnostics": ([vp, vp, vp, vp], i),
         "dgk_ensemble_create": ([vp, i] + [vp] * 5 + [C.c_char_p, i, i], i),
         "dgk_ensemble_destroy": ([vp], i),
@@ -334,6 +338,22 @@ def divlike_batched(Nq, nelem, typeid, Mat, Mat_tr, in_x, in_y, in_z, out_x, out
 
 def lift_batched(Nq, nelem, Lift, fin, fout, stream=None):
     _check(lib().dgk_lift_hexahedral_batched(Nq, nelem, Lift, fin, fout, stream))
+
+
+def matvec_dir_batched(Nq, nelem, direction, Mat, fin, fout, stream=None):
+    """dirX/dirY/dirZ (direction 0/1/2); device pointers."""
+    _check(lib().dgk_matvec_dir_batched(Nq, nelem, direction, Mat, fin, fout, stream))
+
+
+def gradlike_batched(Nq, nelem, Mat, Mat_tr, fin, out_x, out_y, out_z, stream=None):
+    _check(lib().dgk_gradlike_dirXYZ_batched(Nq, nelem, Mat, Mat_tr, fin, out_x, out_y,
+                                             out_z, stream))
+
+
+def divlike_var5_batched(Nq, nelem, Mat, Mat_tr, in_x, in_y, in_z, out_x, out_y, out_z,
+                         stream=None):
+    _check(lib().dgk_divlike_dirXYZ_var5_batched(Nq, nelem, Mat, Mat_tr, in_x, in_y, in_z,
+                                                 out_x, out_y, out_z, stream))
 
 
 class Ensemble:

@@ -53,6 +53,15 @@ def main():
             ox, oy, oz, ol = O.ref_optr(p, t, ix, iy, iz, fb)
             out["optr"][CS.case_key(p, t)] = {"x": summary(ox), "y": summary(oy),
                                               "z": summary(oz), "lift": summary(ol)}
+    # the OPT1 module's directional kernels and gradlike (the reference's
+    # gradlike z output carries its loop slip, mod_dg_optr_kernel_opt1.F90.erb:208)
+    out["optr2"] = {}
+    for p in range(1, 16):
+        a = CS.optr_inputs(p)[0]
+        res = O.ref_optr2(p, a)
+        out["optr2"][CS.case_key(p)] = {
+            k: summary(v) for k, v in zip(("dirX", "dirY", "dirZ", "gradX", "gradY", "gradZ"),
+                                          res)}
     for (ne, p) in CS.TEND_CASES:
         q, u, v, w = CS.tend_inputs(ne, p)
         for t in ("GENERAL", "OPT1"):

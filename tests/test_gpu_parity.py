@@ -68,6 +68,52 @@ def test_batched_operators_bit_exact(gpu, golden, p, tname):
         assert CS.digest(from_dev(t, (Np, n))) == g[k]["sha256"], k
 
 
+@pytest.mark.parametrize("p", range(1, 16))
+def test_batched_directional_gradlike_var5(gpu, golden, p):
+    """dirX/dirY/dirZ and gradlike x, y bit-exact against the reference;
+    gradlike z against the oracle's stated form (the reference's z loop
+    slips, mod_dg_optr_kernel_opt1.F90.erb:208); var5 against the
+    reference's divlike blocks (its own y index slips, :400)."""
+    torch = gpu
+    say(f"[operators2] p={p}")
+    Nq = p + 1
+    Np = Nq ** 3
+    _, D, _ = dgk.read_operator(p)
+    Dt = np.asfortranarray(D.T)
+    ix, iy, iz, _ = CS.optr_inputs(p)
+    n = CS.OPTR_NELEM
+    g = golden["optr2"][CS.case_key(p)]
+    dD, dDt, da = to_dev(torch, D), to_dev(torch, Dt), to_dev(torch, ix)
+    outs = [torch.zeros(Np * n, dtype=torch.float64, device="cuda") for _ in range(6)]
+    for d, M in ((0, dD), (1, dDt), (2, dDt)):
+        dgk.matvec_dir_batched(Nq, n, d, M.data_ptr(), da.data_ptr(), outs[d].data_ptr())
+    dgk.gradlike_batched(Nq, n, dD.data_ptr(), dDt.data_ptr(), da.data_ptr(),
+                         *(o.data_ptr() for o in outs[3:]))
+    torch.cuda.synchronize()
+    res = [from_dev(o, (Np, n)) for o in outs]
+    for r, k in zip(res[:5], ("dirX", "dirY", "dirZ", "gradX", "gradY")):
+        assert CS.digest(r) == g[k]["sha256"], k
+    for e in range(n):
+        want = O.gradlike(Nq, D, Dt, ix[:, e].copy())[2]
+        assert np.array_equal(res[5][:, e], want)
+    assert CS.digest(res[5]) != g["gradZ"]["sha256"]
+    # var5: fields (block 0, 1, 2, 0, 1) of each of two elements
+    order = [0, 1, 2, 0, 1]
+    v5 = [np.concatenate([a[:, b] for _ in range(2) for b in order]) for a in (ix, iy, iz)]
+    din = [to_dev(torch, a) for a in v5]
+    dout = [torch.zeros(Np * 10, dtype=torch.float64, device="cuda") for _ in range(3)]
+    dgk.divlike_var5_batched(Nq, 2, dD.data_ptr(), dDt.data_ptr(), *(t.data_ptr() for t in din),
+                             *(t.data_ptr() for t in dout))
+    torch.cuda.synchronize()
+    go = golden["optr"][CS.case_key(p, "OPT1")]
+    for t, k in zip(dout, "xyz"):
+        r = from_dev(t, (Np, 10))
+        assert CS.digest(np.asfortranarray(r[:, :3])) == go[k]["sha256"], k
+        assert CS.digest(np.asfortranarray(r[:, 5:8])) == go[k]["sha256"], k
+    with pytest.raises(dgk.DGKError):
+        dgk.matvec_dir_batched(Nq, n, 3, dD.data_ptr(), da.data_ptr(), outs[0].data_ptr())
+
+
 # ---------------------------------------------------------------------------
 # tendency (advect3d_eq_cal_tend)
 

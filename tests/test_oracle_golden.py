@@ -38,6 +38,82 @@ def test_operators(golden, p, tname):
         assert CS.digest(outs[k]) == g[gk]["sha256"], (p, tname, k)
 
 
+def _per_block(fn, nblocks, shape_np):
+    """Runs fn(e) -> tuple of (Np,) arrays for e < nblocks into (Np,nblocks)
+    arrays."""
+    outs = None
+    for e in range(nblocks):
+        r = fn(e)
+        if outs is None:
+            outs = [np.zeros((shape_np, nblocks), order="F") for _ in r]
+        for o, v in zip(outs, r):
+            o[:, e] = v
+    return outs
+
+
+def _gradz_as_shipped(a, Dt, Nq):
+    """What the reference's gradlike z loop computes
+    (mod_dg_optr_kernel_opt1.F90.erb:208-213): its first term is
+    in(i,1,k)*Mat_tr(1,j), the rest in(i,j,kk)*Mat_tr(kk,k), left to right.
+    A model of that slip, used only to show what the reference's output is."""
+    A = a.reshape((Nq, Nq, Nq), order="F")
+    out = np.empty_like(A)
+    for k in range(Nq):
+        for j in range(Nq):
+            s = A[:, 0, k] * Dt[0, j]
+            for kk in range(1, Nq):
+                s = s + A[:, j, kk] * Dt[kk, k]
+            out[:, j, k] = s
+    return out.ravel(order="F")
+
+
+@pytest.mark.parametrize("p", range(1, 16))
+def test_directional_and_gradlike(golden, p):
+    """dirX/dirY/dirZ and gradlike x, y equal the reference bit for bit; the
+    reference's gradlike z output is its loop slip (reproduced exactly by a
+    model of it), not the z derivative the oracle computes."""
+    Nq = p + 1
+    Np = Nq ** 3
+    _, D, _ = O.read_operator(p)
+    Dt = np.asfortranarray(D.T)
+    a = CS.optr_inputs(p)[0]
+    n = a.shape[1]
+    g = golden["optr2"][CS.case_key(p)]
+    dx, dy, dz = _per_block(lambda e: (O.matvec_dir(Nq, 0, D, a[:, e].copy()),
+                                       O.matvec_dir(Nq, 1, Dt, a[:, e].copy()),
+                                       O.matvec_dir(Nq, 2, Dt, a[:, e].copy())), n, Np)
+    assert CS.digest(dx) == g["dirX"]["sha256"]
+    assert CS.digest(dy) == g["dirY"]["sha256"]
+    assert CS.digest(dz) == g["dirZ"]["sha256"]
+    gx, gy, gz = _per_block(lambda e: O.gradlike(Nq, D, Dt, a[:, e].copy()), n, Np)
+    assert CS.digest(gx) == g["gradX"]["sha256"]
+    assert CS.digest(gy) == g["gradY"]["sha256"]
+    assert CS.digest(gz) == CS.digest(dz)  # the intended z derivative
+    assert CS.digest(gz) != g["gradZ"]["sha256"]
+    shipped = _per_block(lambda e: (_gradz_as_shipped(a[:, e], Dt, Nq),), n, Np)[0]
+    assert CS.digest(shipped) == g["gradZ"]["sha256"]
+
+
+@pytest.mark.parametrize("p", range(1, 16))
+def test_divlike_var5(golden, p):
+    """var5 (the intended form) is divlike_dirXYZ in OPT1's order on each of
+    five fields: fed the three golden divlike blocks as fields 1-3 (and
+    again as 4-5), its outputs equal the reference's divlike_dirXYZ_P<p>."""
+    Nq = p + 1
+    Np = Nq ** 3
+    _, D, _ = O.read_operator(p)
+    Dt = np.asfortranarray(D.T)
+    ix, iy, iz, _ = CS.optr_inputs(p)
+    order = [0, 1, 2, 0, 1]
+    v5 = [np.concatenate([a[:, e] for e in order]) for a in (ix, iy, iz)]
+    ox, oy, oz = O.divlike_var5(Nq, D, Dt, *v5)
+    g = golden["optr"][CS.case_key(p, "OPT1")]
+    for out, k in ((ox, "x"), (oy, "y"), (oz, "z")):
+        blocks = np.asfortranarray(out.reshape((Np, 5), order="F")[:, :3])
+        assert CS.digest(blocks) == g[k]["sha256"], (p, k)
+        assert np.array_equal(out[3 * Np:], out[:2 * Np])
+
+
 def test_split_order_is_visible(golden):
     """OPT1's three partial sums (p > 7) round differently from GENERAL's
     running sum, so a restatement that ignored the split would fail the
