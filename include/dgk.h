@@ -96,6 +96,16 @@ int dgk_mesh_setup(int NeX, int NeY, int NeZ, int p, double domx, double domy,
                    int32_t *VMapP, double *normal_fn, double *Escale,
                    double *Fscale, double *pos_en, int32_t *halo_src_map);
 
+/* The same tables built on the device, straight into device arrays (any may
+ * be NULL; x1D is a host array).  Replaces mesh_setup's table loops
+ * (mod_mesh.f90:74-193) including the serial halo numbering (:171-190),
+ * which each face node derives in closed form.  Synchronous on stream. */
+int dgk_mesh_setup_device(int NeX, int NeY, int NeZ, int p, double domx,
+                          double domy, double domz, const double *x1D,
+                          int32_t *VMapM, int32_t *VMapP, double *normal_fn,
+                          double *Escale, double *Fscale, double *pos_en,
+                          int32_t *halo_src_map, void *stream);
+
 /* ---- device-resident solver ------------------------------------------- */
 /* Creates a solver from the reference's mesh-level arrays (the argument list
  * of advect3d_eq_cal_tend, mod_advect3d_eq.f90:41-66, plus the halo map of

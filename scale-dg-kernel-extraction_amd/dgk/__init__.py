@@ -38,6 +38,7 @@ EXPORTS = (
     "dgk_ensemble_step", "dgk_ensemble_run", "dgk_ensemble_time_stages",
     "dgk_ensemble_tune", "dgk_ensemble_query", "dgk_autotune", "dgk_diagnostics",
     "dgk_matvec_dir_batched", "dgk_gradlike_dirXYZ_batched", "dgk_divlike_dirXYZ_var5_batched",
+    "dgk_mesh_setup_device",
 )
 
 
@@ -87,6 +88,7 @@ def lib():
         "dgk_divlike_dirXYZ_batched": ([i, i, i] + [vp] * 9, i),
         "dgk_lift_hexahedral_batched": ([i, i, vp, vp, vp, vp], i),
         "dgk_autotune": ([vp, d, i, vp, vp], i),
+        "dgk_mesh_setup_device": ([i, i, i, i, d, d, d] + [vp] * 9, i),
         "dgk_matvec_dir_batched": ([i, i, i, vp, vp, vp, vp], i),
         "dgk_gradlike_dirXYZ_batched": ([i, i] + [vp] * 7, i),
         "dgk_divlike_dirXYZ_var5_batched": ([i, i] + [vp] * 9, i),
@@ -338,6 +340,16 @@ def divlike_batched(Nq, nelem, typeid, Mat, Mat_tr, in_x, in_y, in_z, out_x, out
 
 def lift_batched(Nq, nelem, Lift, fin, fout, stream=None):
     _check(lib().dgk_lift_hexahedral_batched(Nq, nelem, Lift, fin, fout, stream))
+
+
+def mesh_setup_device(NeX, NeY, NeZ, p, x1D, ptrs, dom=(1.0, 1.0, 1.0), stream=None):
+    """dgk_mesh_setup_device: ptrs maps VMapM, VMapP, normal_fn, Escale, Fscale,
+    pos_en, halo_src_map to device addresses (missing keys: not built)."""
+    x1D = np.ascontiguousarray(x1D, dtype=np.float64)
+    keys = ("VMapM", "VMapP", "normal_fn", "Escale", "Fscale", "pos_en", "halo_src_map")
+    _check(lib().dgk_mesh_setup_device(NeX, NeY, NeZ, p, float(dom[0]), float(dom[1]),
+                                       float(dom[2]), _p(x1D), *(ptrs.get(k) for k in keys),
+                                       stream))
 
 
 def matvec_dir_batched(Nq, nelem, direction, Mat, fin, fout, stream=None):

@@ -111,3 +111,23 @@ def test_autotune_keeps_state_and_bits(gpu, p, ne):
     om.run(*F, dt, nstep, O.OPT1)
     assert np.array_equal(s.get_field(), F[0][:, :om.Ne])
     s.close()
+
+
+@pytest.mark.parametrize("ne,p,dom", [((3, 4, 5), 3, (1.0, 1.0, 1.0)), ((1, 2, 3), 7, (2.0, 0.5, 1.5)),
+                                      ((4, 1, 1), 2, (1.0, 3.0, 0.25)), ((2, 2, 2), 11, (1.0, 1.0, 1.0))])
+def test_device_mesh_build_equals_host(gpu, ne, p, dom):
+    """dgk_mesh_setup_device writes the host build's tables bit for bit,
+    including the halo numbering of the reference's serial pass."""
+    torch = gpu
+    say(f"[mesh device] {ne} p={p} dom={dom}")
+    x1D, _, _ = dgk.read_operator(p)
+    m = dgk.mesh_setup(*ne, p, x1D=x1D, dom=dom)
+    keys = {"VMapM": torch.int32, "VMapP": torch.int32, "halo_src_map": torch.int32,
+            "normal_fn": torch.float64, "Escale": torch.float64, "Fscale": torch.float64,
+            "pos_en": torch.float64}
+    dev = {k: torch.full((m[k].size,), -7, dtype=t, device="cuda") for k, t in keys.items()}
+    dgk.mesh_setup_device(*ne, p, x1D, {k: v.data_ptr() for k, v in dev.items()}, dom=dom)
+    for k in keys:
+        got = dev[k].cpu().numpy()
+        want = np.asarray(m[k]).ravel(order="F")
+        assert got.dtype == want.dtype and np.array_equal(got, want), k
