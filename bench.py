@@ -14,7 +14,11 @@ bounded sample of the workload.
 
     python bench.py [--steps K] [--warmup W] [--workload c3_64cube_p7]
                     [--variant factored|exact|fma|...] [--no-cpu-baseline]
-                    [--copy-calibration]
+                    [--copy-calibration] [--no-error] [--no-extra]
+
+The line also carries the measured variant's rel-L2 and max error against
+the FP64 oracle's bits after the workload's nstep, and (unless --no-extra)
+the p=15 16^3 workload and the 64-instance ensemble.
 
 --copy-calibration first copies 1 GiB device to device (hipMemcpy's copy
 kernel) three times, so that a rocprofv3 --pmc run of this script also
@@ -114,6 +118,90 @@ def traffic_from_profiles(workload, variant):
     return t.get("bytes_per_launch")
 
 
+def _fields(spec, x1D):
+    ne, p = spec["ne"], spec["p"]
+    m = dgk.mesh_setup(*ne, p, x1D=x1D, tables=False)
+    q = ic.initial_field(spec["ic"], m["pos_en"], spec["seed"])
+    return q
+
+
+def error_vs_oracle(workload, flags, x1D, D, L, spec):
+    """rel-L2 and max error of the variant against the bit-exact variant
+    after the workload's nstep, with the mass drift of both (device
+    reductions)."""
+    ne, p, vel, nstep = spec["ne"], spec["p"], spec["vel"], spec["nstep"]
+    q = _fields(spec, x1D)
+    dt = ic.cfl_dt(p, vel, ne)
+    runs = []
+    for f in (flags, VARIANTS["exact"]):
+        s = dgk.Solver.structured(*ne, p, D, L, flags=f)
+        s.set_fields(q, *(np.full_like(q, c) for c in vel))
+        m0 = s.diagnostics(x1D)
+        s.run(dt, nstep)
+        runs.append((s, m0))
+    (a, a0), (b, b0) = runs
+    d = a.diagnostics(x1D, ref=b)
+    db = b.diagnostics(x1D)
+    out = {"nstep": nstep, "rel_l2": d["rel_l2"], "max_abs": d["max_err"],
+           "max_rel": d["max_err"] / max(abs(db["min"]), abs(db["max"])),
+           "mass_drift_rel": abs(d["mass"] - a0["mass"]) / a0["abs_mass"],
+           "oracle_mass_drift_rel": abs(db["mass"] - b0["mass"]) / b0["abs_mass"],
+           "reference": "exact variant (bit-identical to the FP64 oracle)"}
+    a.close()
+    b.close()
+    return out
+
+
+def single_workload(workload, variants, nstage=30):
+    """Stage kernel time of a few variants on another workload (after
+    autotuning each), in GDOF-updates/s."""
+    spec = ic.WORKLOADS[workload]
+    ne, p, vel = spec["ne"], spec["p"], spec["vel"]
+    x1D, D, L = dgk.read_operator(p)
+    q = _fields(spec, x1D)
+    dt = ic.cfl_dt(p, vel, ne)
+    res = {"NeX": ne[0], "PolyOrder": p}
+    for name in variants:
+        s = dgk.Solver.structured(*ne, p, D, L, flags=VARIANTS[name])
+        s.set_fields(q, *(np.full_like(q, c) for c in vel))
+        var, _ = s.autotune(dt, nrep=6)
+        ms = s.time_stages(dt, nstage)
+        bpd = 24 if VARIANTS[name] & dgk.FLAG_CONST_VELOCITY else BYTES_PER_DOF
+        res[name] = {"gdof_updates_per_s": round(s.Ne * s.Np / (ms * 1e-3) / 1e9, 2),
+                     "kernel_us": round(ms * 1e3, 1), "variant": var,
+                     "hbm_frac": round(bpd * s.Ne * s.Np / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        s.close()
+        log(f"{workload} {name}: {res[name]['gdof_updates_per_s']} GDOF-updates/s")
+    return res
+
+
+def ensemble_workload(nstage=30):
+    """The 64-instance ensemble (orders 3, 5, 7, 11, uneven meshes, uniform
+    per-instance velocity): one persistent launch per stage, grid split
+    tuned per order class."""
+    inst = ic.ensemble_workload(**{k: v for k, v in ic.ENSEMBLE.items() if k in ("n", "seed")})
+    ens = dgk.Ensemble(inst, flags=dgk.FLAG_CONST_VELOCITY | dgk.FLAG_FACTORED_LIFT)
+    for i, sp in enumerate(inst):
+        x1D, _, _ = dgk.read_operator(sp["p"])
+        m = dgk.mesh_setup(*sp["ne"], sp["p"], x1D=x1D, tables=False)
+        qi = ic.blob(m["pos_en"], sp["centre"], sp["width"])
+        ens.set_fields(i, qi, *(np.full_like(qi, c) for c in sp["vel"]))
+    ens.set_dt([sp["dt"] for sp in inst])
+    ens.tune(3)
+    ms = ens.time_stages(nstage)
+    t0 = time.perf_counter()
+    ens.run(5)
+    el = time.perf_counter() - t0
+    dof = ens.query(0)
+    res = {"instances": len(inst), "DOF": dof, "classes": ens.classes,
+           "gdof_updates_per_s_kernel": round(dof / (ms * 1e-3) / 1e9, 2),
+           "gdof_updates_per_s_graph": round(dof * 15 / el / 1e9, 2),
+           "variant": "FP64, factored lift, uniform velocity per instance"}
+    ens.close()
+    log(f"ensemble: {res['gdof_updates_per_s_kernel']} GDOF-updates/s per stage")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -125,11 +213,21 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-steps", type=int, default=5)
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--no-error", action="store_true",
+                    help="skip the nstep error run against the bit-exact variant")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the p=15 and ensemble side measurements")
+    ap.add_argument("--ensemble-only", action="store_true",
+                    help="measure only the 64-instance ensemble (one JSON line)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
               "sharding of the mesh is a design, not a measured path", file=sys.stderr)
         sys.exit(2)
+
+    if args.ensemble_only:
+        print(json.dumps({"workload": "c4_ensemble_64", **ensemble_workload()}), flush=True)
+        return
 
     if args.copy_calibration:
         import torch
@@ -153,7 +251,13 @@ def main():
     dt = ic.cfl_dt(p, vel, ne)
     log(f"initial field generated in {time.perf_counter() - t0:.1f} s")
     s = dgk.Solver.structured(*ne, p, D, L, flags=VARIANTS[args.variant])
-    kernel_name = "pencil_kernel" if s.kernel_kind == "pencil" else "stage_kernel"
+    if s.kernel_kind != "pencil":
+        kernel_name = "stage_kernel"
+    elif p + 1 in (6, 12, 16) or s.variant & 4:
+        kernel_name = "pencil_group_kernel"
+    else:
+        kernel_name = "pencil_kernel"
+    bytes_per_dof = 24 if VARIANTS[args.variant] & dgk.FLAG_CONST_VELOCITY else BYTES_PER_DOF
     s.set_fields(q, u, v, w)
     w3 = np.einsum("i,j,k->ijk", *(lgl_weights(x1D),) * 3).ravel(order="F")
     J = (1.0 / ne[0]) * (1.0 / ne[1]) * (1.0 / ne[2]) / 8.0
@@ -178,7 +282,7 @@ def main():
     # solver's own stream
     nstage = max(30, 3 * args.steps)
     ms_stage = s.time_stages(dt, nstage)
-    bytes_launch = BYTES_PER_DOF * dof
+    bytes_launch = bytes_per_dof * dof
     achieved = bytes_launch / (ms_stage * 1e-3) / 1e9
     qf = s.get_field()
     finite = bool(np.isfinite(qf).all())
@@ -195,6 +299,21 @@ def main():
     drift = abs(mass(qf, w3, J)[0] - mass0) / mass_abs
     s.close()
     log(f"stage kernel {ms_stage * 1e3:.1f} us/launch -> {achieved:.0f} GB/s algorithmic")
+
+    # error against the FP64 oracle after the workload's nstep: the exact
+    # variant gives the oracle's bits (the parity tests show it), so the
+    # measured variant is compared with it on the device, from the same
+    # initial field
+    err = None
+    if not args.no_error:
+        err = error_vs_oracle(args.workload, VARIANTS[args.variant], x1D, D, L, spec)
+        log(f"after {err['nstep']} steps: rel-L2 {err['rel_l2']:.3e}, max {err['max_abs']:.3e}")
+
+    extra = {}
+    if not args.no_extra:
+        extra["c5_16cube_p15"] = single_workload("c5_16cube_p15", ("exact", "factored",
+                                                                   "factored_constvel"))
+        extra["c4_ensemble_64"] = ensemble_workload()
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -226,6 +345,8 @@ def main():
                      "algorithmic_bytes_per_launch": bytes_launch},
         "cpu_baseline": cpu,
         "checks": {"finite": finite, "mass_drift_rel": drift},
+        "error_vs_fp64_oracle": err,
+        "other_workloads": extra or None,
         "exact_variant": None if ms_exact is None else {
             "kernel_us": round(ms_exact * 1e3, 2),
             "gdof_updates_per_s": round(dof / (ms_exact * 1e-3) / 1e9, 2),
