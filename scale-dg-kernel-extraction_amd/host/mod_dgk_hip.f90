@@ -16,13 +16,16 @@ module mod_dgk_hip
   integer, public, parameter :: DGK_OK = 0
   integer, public, parameter :: DGK_TYPE_GENERAL = 1, DGK_TYPE_OPT1 = 2, DGK_TYPE_HIP = 3
   integer, public, parameter :: DGK_FLAG_FMA = 1, DGK_FLAG_FORCE_TABLES = 2, &
-       DGK_FLAG_NODE_KERNEL = 4, DGK_FLAG_FACTORED_LIFT = 8
+       DGK_FLAG_NODE_KERNEL = 4, DGK_FLAG_FACTORED_LIFT = 8, DGK_FLAG_CONST_VELOCITY = 16
   integer, public, parameter :: DGK_FIELD_Q = 0, DGK_FIELD_Q0 = 1, DGK_FIELD_DQDT = 2
 
   public :: dgk_optr_kernel_setup, dgk_read_operator, dgk_mesh_sizes, dgk_mesh_setup
   public :: dgk_create, dgk_create_structured, dgk_destroy
   public :: dgk_set_fields, dgk_get_field, dgk_cal_tend, dgk_rk_stage, dgk_step, dgk_run
   public :: dgk_synchronize, dgk_advect3d_eq_cal_tend, dgk_error_message
+  public :: dgk_autotune, dgk_diagnostics
+  public :: dgk_ensemble_create, dgk_ensemble_destroy, dgk_ensemble_set_fields
+  public :: dgk_ensemble_get_field, dgk_ensemble_set_dt, dgk_ensemble_run, dgk_ensemble_tune
 
   interface
     integer(c_int) function dgk_optr_kernel_setup(evaltype, type_id) bind(C, name="dgk_optr_kernel_setup")
@@ -130,6 +133,75 @@ module mod_dgk_hip
       integer(c_int), intent(in) :: VMapM(*), VMapP(*)
       real(c_double), intent(in) :: normal_fn(*), Escale(*), Fscale(*)
       integer(c_int), value :: Nq, Np, NfpTot, Ne, NeA, type_id
+    end function
+
+    integer(c_int) function dgk_autotune(solver, dt, nrep, variant, ms) bind(C, name="dgk_autotune")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: solver
+      real(c_double), value :: dt
+      integer(c_int), value :: nrep
+      integer(c_int), intent(out) :: variant
+      real(c_double), intent(out) :: ms
+    end function
+
+    ! out(8): min, max, mass, |q| mass, sum q^2, sum (q-ref)^2, sum ref^2, max |q-ref|;
+    ! ref is a device address (c_null_ptr for none)
+    integer(c_int) function dgk_diagnostics(solver, w1D, ref, out) bind(C, name="dgk_diagnostics")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: solver
+      real(c_double), intent(in) :: w1D(*)
+      type(c_ptr), value :: ref
+      real(c_double), intent(out) :: out(8)
+    end function
+
+    ! the ensemble: n independent runs advanced together
+    integer(c_int) function dgk_ensemble_create(ens, n, p, NeX, NeY, NeZ, dom, operator_dir, &
+         type_id, flags) bind(C, name="dgk_ensemble_create")
+      import :: c_int, c_ptr, c_char
+      type(c_ptr), intent(out) :: ens
+      integer(c_int), value :: n, type_id, flags
+      integer(c_int), intent(in) :: p(*), NeX(*), NeY(*), NeZ(*)
+      type(c_ptr), value :: dom
+      character(kind=c_char), intent(in) :: operator_dir(*)
+    end function
+
+    integer(c_int) function dgk_ensemble_destroy(ens) bind(C, name="dgk_ensemble_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ens
+    end function
+
+    integer(c_int) function dgk_ensemble_set_fields(ens, i, q, u, v, w) &
+         bind(C, name="dgk_ensemble_set_fields")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ens
+      integer(c_int), value :: i
+      real(c_double), intent(in) :: q(*), u(*), v(*), w(*)
+    end function
+
+    integer(c_int) function dgk_ensemble_get_field(ens, i, q) bind(C, name="dgk_ensemble_get_field")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ens
+      integer(c_int), value :: i
+      real(c_double), intent(out) :: q(*)
+    end function
+
+    integer(c_int) function dgk_ensemble_set_dt(ens, dt) bind(C, name="dgk_ensemble_set_dt")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ens
+      real(c_double), intent(in) :: dt(*)
+    end function
+
+    integer(c_int) function dgk_ensemble_run(ens, nstep) bind(C, name="dgk_ensemble_run")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ens
+      integer(c_int), value :: nstep
+    end function
+
+    integer(c_int) function dgk_ensemble_tune(ens, nrep, ms) bind(C, name="dgk_ensemble_tune")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ens
+      integer(c_int), value :: nrep
+      real(c_double), intent(out) :: ms(*)
     end function
 
     type(c_ptr) function dgk_last_error() bind(C, name="dgk_last_error")
