@@ -55,6 +55,9 @@ VARIANTS = {
     "factored_constvel": dgk.FLAG_FACTORED_LIFT | dgk.FLAG_CONST_VELOCITY,
     "exact_constvel": dgk.FLAG_CONST_VELOCITY,
     "fma_node": dgk.FLAG_FMA | dgk.FLAG_NODE_KERNEL,
+    "fma_factored_constvel": dgk.FLAG_FMA | dgk.FLAG_FACTORED_LIFT | dgk.FLAG_CONST_VELOCITY,
+    # FP32 split contractions (pencil kernel, Nq 2, 4, 8), FP64 state
+    "mixed": dgk.FLAG_MIXED,
 }
 
 
@@ -152,6 +155,33 @@ def error_vs_oracle(workload, flags, x1D, D, L, spec):
     return out
 
 
+def mixed_workload(workload, x1D, D, L, spec, nstage=30):
+    """The FP32-split variant (DGK_FLAG_MIXED): stage time and its error
+    against the oracle's bits after the workload's nstep."""
+    ne, p, vel = spec["ne"], spec["p"], spec["vel"]
+    if p + 1 not in (2, 4, 8):
+        return None
+    q = _fields(spec, x1D)
+    dt = ic.cfl_dt(p, vel, ne)
+    s = dgk.Solver.structured(*ne, p, D, L, flags=VARIANTS["mixed"])
+    s.set_fields(q, *(np.full_like(q, c) for c in vel))
+    del q
+    s.run(dt, 1)
+    ms = s.time_stages(dt, nstage)
+    s.close()
+    err = error_vs_oracle(workload, VARIANTS["mixed"], x1D, D, L, spec)
+    dof = ne[0] * ne[1] * ne[2] * (p + 1) ** 3
+    res = {"gdof_updates_per_s": round(dof / (ms * 1e-3) / 1e9, 2),
+           "kernel_us": round(ms * 1e3, 1),
+           "hbm_frac": round(BYTES_PER_DOF * dof / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+           "error_vs_fp64_oracle": err,
+           "variant": "FP32 split contractions and lift (hi*hi and hi*lo+lo*hi FP32 sums), "
+                      "FP64 flux differences, residual and RK"}
+    log(f"mixed: {res['gdof_updates_per_s']} GDOF-updates/s, rel-L2 {err['rel_l2']:.2e} after "
+        f"{err['nstep']} steps")
+    return res
+
+
 def single_workload(workload, variants, nstage=30):
     """Stage kernel time of a few variants on another workload (after
     autotuning each), in GDOF-updates/s."""
@@ -180,7 +210,8 @@ def ensemble_workload(nstage=30):
     per-instance velocity): one persistent launch per stage, grid split
     tuned per order class."""
     inst = ic.ensemble_workload(**{k: v for k, v in ic.ENSEMBLE.items() if k in ("n", "seed")})
-    ens = dgk.Ensemble(inst, flags=dgk.FLAG_CONST_VELOCITY | dgk.FLAG_FACTORED_LIFT)
+    ens = dgk.Ensemble(inst, flags=dgk.FLAG_CONST_VELOCITY | dgk.FLAG_FACTORED_LIFT
+                       | dgk.FLAG_FMA)
     for i, sp in enumerate(inst):
         x1D, _, _ = dgk.read_operator(sp["p"])
         m = dgk.mesh_setup(*sp["ne"], sp["p"], x1D=x1D, tables=False)
@@ -196,7 +227,7 @@ def ensemble_workload(nstage=30):
     res = {"instances": len(inst), "DOF": dof, "classes": ens.classes,
            "gdof_updates_per_s_kernel": round(dof / (ms * 1e-3) / 1e9, 2),
            "gdof_updates_per_s_graph": round(dof * 15 / el / 1e9, 2),
-           "variant": "FP64, factored lift, uniform velocity per instance"}
+           "variant": "FP64, fused multiply-adds, factored lift, uniform velocity per instance"}
     ens.close()
     log(f"ensemble: {res['gdof_updates_per_s_kernel']} GDOF-updates/s per stage")
     return res
@@ -311,8 +342,10 @@ def main():
 
     extra = {}
     if not args.no_extra:
+        extra["c3_64cube_p7_mixed"] = mixed_workload(args.workload, x1D, D, L, spec)
         extra["c5_16cube_p15"] = single_workload("c5_16cube_p15", ("exact", "factored",
-                                                                   "factored_constvel"))
+                                                                   "factored_constvel",
+                                                                   "fma_factored_constvel"))
         extra["c4_ensemble_64"] = ensemble_workload()
 
     cpu = None

@@ -57,6 +57,14 @@ extern "C" {
                                       24 B per DOF-update instead of 48; same
                                       results (pencil kernel only)          */
 
+#define DGK_FLAG_MIXED 0x20         /* directional contractions and lift in
+                                      FP32 with hi/lo split operators and
+                                      operands (FP32 sums of hi*hi and of
+                                      hi*lo + lo*hi); face-flux differences,
+                                      residual and RK in FP64.  Implies the
+                                      factored lift; one-wavefront pencil
+                                      kernel only (Nq 2, 4, 8), no FMA flag */
+
 /* Fields addressable through dgk_get_field / dgk_field_ptr. */
 #define DGK_FIELD_Q 0
 #define DGK_FIELD_Q0 1

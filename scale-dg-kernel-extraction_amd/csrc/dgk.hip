@@ -199,8 +199,10 @@ template <int NQ, int ORDER, bool FMA, bool FACT, bool NT, bool CV>
 void launch_pencil(const StageKernel &k, const StageParams &P, const HostOps &H,
                    const double *Lift, hipStream_t st) {
   (void)Lift;
+  const double *dr = ORDER == kMixed ? H.Drow32 : H.Drow;
+  const double *lf = ORDER == kMixed ? H.lfac32 : H.lfac;
   hipLaunchKernelGGL((pencil_kernel<NQ, ORDER, FMA, FACT, NT, CV>), dim3((unsigned)k.grid),
-                     dim3(64), k.lds, st, P, H.Drow, H.LiftP, H.lfac);
+                     dim3(64), k.lds, st, P, dr, H.LiftP, lf);
 }
 
 template <int NQ, int ORDER, bool FMA, bool FACT, bool NT, bool CV>
@@ -246,6 +248,10 @@ StageKernel pick_pencil_fact(bool fact, bool nt, bool cv) {
 }
 template <int NQ>
 StageKernel pick_pencil_nq(int order, bool fma, bool fact, bool nt, bool cv) {
+  if (order == kMixed) {  // FP32 split contractions: plain products, factored lift only
+    if (fma || !fact) return StageKernel{};
+    return pick_pencil_nt<NQ, kMixed, false, true>(nt, cv);
+  }
   if (order == kGeneral)
     return fma ? pick_pencil_fact<NQ, kGeneral, true>(fact, nt, cv)
                : pick_pencil_fact<NQ, kGeneral, false>(fact, nt, cv);
@@ -260,6 +266,7 @@ bool pencil_nq(int Nq) {
 // Nq 2, 4, 8: pencil_kernel, or with `wide` the group kernel (one
 // wavefront, lean layout); Nq 6, 12, 16: the group kernel.
 StageKernel pick_pencil(int Nq, int order, bool fma, bool fact, bool nt, bool cv, bool wide) {
+  if (order == kMixed && (Nq == 6 || Nq == 12 || Nq == 16 || wide)) return StageKernel{};
   if (Nq == 6 || Nq == 12 || Nq == 16 || wide)
     return pick_pencil_group(Nq, order, fma, fact, cv, group_epb(Nq, wide));
   switch (Nq) {
@@ -341,6 +348,7 @@ struct dgk_solver {
   double *q[2] = {nullptr, nullptr};
   double *q0 = nullptr, *u = nullptr, *v = nullptr, *w = nullptr, *dqdt = nullptr;
   double *D = nullptr, *Lift = nullptr, *lfac = nullptr, *Drow = nullptr, *LiftP = nullptr;
+  double *Drow32 = nullptr, *lfac32 = nullptr;  // FP32 split operators (mixed precision)
   int32_t *nbr = nullptr, *vmapM = nullptr, *vmapP = nullptr, *halo = nullptr;
   double *nrm = nullptr, *fscale = nullptr, *escale = nullptr;
   double fs[6] = {0, 0, 0, 0, 0, 0}, es[3] = {0, 0, 0};
@@ -384,13 +392,16 @@ int default_pencil_variant(int Nq) {
 // order x workgroup shape (Nq 16 has one shape).
 int pencil_variants(const dgk_solver *s) {
   if (!s->sk.pencil) return 1;
-  return s->Nq == 16 ? kPencilWide : kPencilVariants;
+  return (s->Nq == 16 || s->order == kMixed) ? kPencilWide : kPencilVariants;
 }
 
 int configure_launch(dgk_solver *s) {
   const bool use_pencil =
       s->structured && !(s->flags & DGK_FLAG_NODE_KERNEL) && pencil_nq(s->Nq);
   const int var = s->variant >= 0 ? s->variant : default_pencil_variant(s->Nq);
+  if (s->order == kMixed && !use_pencil)
+    return fail(DGK_ERR_ARG, "DGK_FLAG_MIXED needs the one-wavefront pencil kernel (Nq 2, 4 "
+                             "or 8, compressed connectivity) without DGK_FLAG_FMA");
   if ((s->flags & DGK_FLAG_CONST_VELOCITY) && !use_pencil)
     return fail(DGK_ERR_ARG, "DGK_FLAG_CONST_VELOCITY needs the pencil kernel (Nq 2, 4, 6, 8, "
                              "12 or 16, compressed connectivity)");
@@ -400,6 +411,9 @@ int configure_launch(dgk_solver *s) {
                         (var & kPencilWide) != 0);
   else
     s->sk = pick_stage(s->Nq, s->order, s->fma, s->structured);
+  if (!s->sk.kernel && s->order == kMixed)
+    return fail(DGK_ERR_ARG, "DGK_FLAG_MIXED needs the one-wavefront pencil kernel (Nq 2, 4 "
+                             "or 8, compressed connectivity) without DGK_FLAG_FMA");
   if (!s->sk.kernel) return fail(DGK_ERR_ORDER, "Nq %d not supported", s->Nq);
   if ((s->flags & DGK_FLAG_FACTORED_LIFT) && !use_pencil)
     return fail(DGK_ERR_ARG, "DGK_FLAG_FACTORED_LIFT needs the pencil kernel (Nq 2, 4, 6, 8, "
@@ -589,7 +603,8 @@ int finish_create(dgk_solver *s, const double *D1D, const double *Lift) {
   const size_t nq2 = (size_t)Nq * Nq, nlift = (size_t)6 * Np;
   if ((rc = dalloc(&s->D, nq2)) || (rc = dalloc(&s->Lift, nlift)) ||
       (rc = dalloc(&s->lfac, 2 * (size_t)Nq)) || (rc = dalloc(&s->Drow, nq2)) ||
-      (rc = dalloc(&s->LiftP, nlift)))
+      (rc = dalloc(&s->LiftP, nlift)) || (rc = dalloc(&s->Drow32, nq2)) ||
+      (rc = dalloc(&s->lfac32, 2 * (size_t)Nq)))
     return rc;
   // pair-packed Lift for the pencil kernel: line pl = i + Nq*j holds the
   // 6*Nq values v = 6k + f, Lift(i,j,k,f); pair m of line pl at
@@ -619,6 +634,26 @@ int finish_create(dgk_solver *s, const double *D1D, const double *Lift) {
   }
   HIPOK(hipMemcpy(s->lfac, lf.data(), lf.size() * sizeof(double), hipMemcpyHostToDevice));
   s->ops.lfac = s->lfac;
+  // FP32 split copies: x = hi + lo, hi = fl32(x), lo = fl32(x - hi)
+  {
+    std::vector<float> d32(2 * nq2), l32(4 * (size_t)Nq);
+    for (int k = 0; k < Nq; ++k)
+      for (int l = 0; l < Nq; ++l) {
+        const double x = dr[(size_t)k * Nq + l];
+        const float hi = (float)x;
+        d32[(size_t)k * 2 * Nq + l] = hi;
+        d32[(size_t)k * 2 * Nq + Nq + l] = (float)(x - (double)hi);
+      }
+    for (int t = 0; t < 2 * Nq; ++t) {
+      const float hi = (float)lf[t];
+      l32[t] = hi;
+      l32[2 * Nq + t] = (float)(lf[t] - (double)hi);
+    }
+    HIPOK(hipMemcpy(s->Drow32, d32.data(), d32.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIPOK(hipMemcpy(s->lfac32, l32.data(), l32.size() * sizeof(float), hipMemcpyHostToDevice));
+    s->ops.Drow32 = s->Drow32;
+    s->ops.lfac32 = s->lfac32;
+  }
   if ((rc = alloc_fields(s))) return rc;
   if ((rc = configure_launch(s))) return rc;
   HIPOK(hipStreamSynchronize(s->stream));
@@ -643,7 +678,7 @@ int dgk_destroy(dgk_solver *s) {
   drop_graphs(s);
   void *bufs[] = {s->q[0], s->q[1], s->q0, s->u, s->v, s->w, s->dqdt, s->D, s->Lift, s->lfac, s->Drow, s->LiftP,
                   s->nbr, s->vmapM, s->vmapP, s->halo, s->nrm, s->fscale, s->escale,
-                  s->part, s->mm, s->diag};
+                  s->part, s->mm, s->diag, s->Drow32, s->lfac32};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -686,9 +721,10 @@ int dgk_create(dgk_solver **out, int Nq, int Ne, int NeA, int NfpTot, const doub
   s->Ne = Ne;
   s->NeA = NeA;
   s->NhaloNode = NhaloNode;
-  s->order = order;
+  s->order = (flags & DGK_FLAG_MIXED) ? kMixed : order;
   s->fma = (flags & DGK_FLAG_FMA) != 0;
-  s->flags = flags;
+  // the mixed form lifts with the 1D vectors
+  s->flags = (flags & DGK_FLAG_MIXED) ? (flags | DGK_FLAG_FACTORED_LIFT) : flags;
   int rc = DGK_OK;
   if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
     delete s;
@@ -740,9 +776,10 @@ int dgk_create_structured(dgk_solver **out, int NeX, int NeY, int NeZ, int p, do
   s->Ne = sz[4];
   s->NeA = sz[5];
   s->NhaloNode = sz[6];
-  s->order = order;
+  s->order = (flags & DGK_FLAG_MIXED) ? kMixed : order;
   s->fma = (flags & DGK_FLAG_FMA) != 0;
-  s->flags = flags;
+  // the mixed form lifts with the 1D vectors
+  s->flags = (flags & DGK_FLAG_MIXED) ? (flags | DGK_FLAG_FACTORED_LIFT) : flags;
   s->structured = true;
   s->NeX = NeX;
   s->NeY = NeY;

@@ -136,14 +136,63 @@ __device__ __forceinline__ void sgpr_row(const double *Drow, int k, double after
                                           ((unsigned long long)(unsigned)dw[2 * l + 1] << 32));
 }
 
+// FP32 row k of the split operator: the 2*NQ dwords of a row hold D_hi(k,.)
+// then D_lo(k,.) (the same row length in dwords as the FP64 operator).
+template <int NQ>
+__device__ __forceinline__ void sgpr_row_split(const double *Drow, int k, double after,
+                                               float (&hi)[NQ], float (&lo)[NQ]) {
+  double r[NQ];
+  sgpr_row<NQ>(Drow, k, after, r);
+#pragma unroll
+  for (int l = 0; l < NQ; ++l) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, r[l]);
+    const unsigned w0 = (unsigned)b, w1 = (unsigned)(b >> 32);
+    // dword 2l and 2l+1 of the row
+    const int d0 = 2 * l, d1 = 2 * l + 1;
+    if (d0 < NQ) hi[d0] = __builtin_bit_cast(float, w0); else lo[d0 - NQ] = __builtin_bit_cast(float, w0);
+    if (d1 < NQ) hi[d1] = __builtin_bit_cast(float, w1); else lo[d1 - NQ] = __builtin_bit_cast(float, w1);
+  }
+}
+
+// x = hi + lo with hi = fl32(x), lo = fl32(x - hi).
+__device__ __forceinline__ void split_f64(double x, float &hi, float &lo) {
+  hi = (float)x;
+  lo = (float)(x - (double)hi);
+}
+
+// Rows of D in flight during a sweep: row k is loaded once output
+// k - kSweepDepth is done (kSweepDepth rows of SGPRs live).  A depth of 2,
+// which overlaps a row's load with the previous output's arithmetic, measured
+// no faster at p = 7, 11, 15 than 1.
+constexpr int kSweepDepth = 1;
+
 // out[k] = sum_l D(k,l) f[l] in the reference's order, one SGPR row at a time.
+// ORDER == kMixed: the FP32 split form (Drow then holds the split rows).
 template <int NQ, int ORDER, bool FMA, bool MAT_FIRST>
 __device__ __forceinline__ void sweep(const double *Drow, const double (&f)[NQ],
                                       double (&out)[NQ]) {
+  if constexpr (ORDER == kMixed) {
+    float fh[NQ], fl[NQ];
+#pragma unroll
+    for (int l = 0; l < NQ; ++l) split_f64(f[l], fh[l], fl[l]);
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) {
+      float dh[NQ], dl[NQ];
+      sgpr_row_split<NQ>(Drow, k, k >= kSweepDepth ? out[k - kSweepDepth] : f[0], dh, dl);
+      float acc = 0.0f, cor = 0.0f;
+#pragma unroll
+      for (int l = 0; l < NQ; ++l) {
+        acc = __builtin_fmaf(dh[l], fh[l], acc);
+        cor = __builtin_fmaf(dh[l], fl[l], __builtin_fmaf(dl[l], fh[l], cor));
+      }
+      out[k] = (double)acc + (double)cor;
+    }
+    return;
+  }
 #pragma unroll
   for (int k = 0; k < NQ; ++k) {
     double r[NQ];
-    sgpr_row<NQ>(Drow, k, k ? out[k - 1] : f[0], r);
+    sgpr_row<NQ>(Drow, k, k >= kSweepDepth ? out[k - kSweepDepth] : f[0], r);
     // x: Mat(i,l)*in(l); y,z: in(l)*Mat_tr(l,j) -- products commute
     out[k] = contract<NQ, ORDER, FMA>([&](int l) { return r[l]; }, [&](int l) { return f[l]; });
   }
@@ -374,7 +423,24 @@ __device__ __forceinline__ void pencil_element(const StageParams &P, const doubl
     const double f3 = Bf[2 * FPAD + pidx<NQ>(j + NQ * k)];  // in(j,k,4)
     const double f1 = Bf[3 * FPAD + pidx<NQ>(j + NQ * k)];  // in(j,k,2)
     double L[6];
-    if constexpr (FACT_LIFT) {
+    if constexpr (ORDER == kMixed) {
+      // lfac holds the split vectors as FP32: [l_-hi, l_+hi, l_-lo, l_+lo]
+      static_assert(FACT_LIFT, "the mixed form lifts with the 1D vectors");
+      const float *lf = reinterpret_cast<const float *>(lfac);
+      const int li[6] = {j, NQ + i, NQ + j, i, k, NQ + k};
+      const double fv[6] = {f0, f1, f2, f3, flux4, flux5};
+      float acc = 0.0f, cor = 0.0f;
+#pragma unroll
+      for (int f = 0; f < 6; ++f) {
+        float fh, fl;
+        split_f64(fv[f], fh, fl);
+        const float lh = lf[li[f]], ll = lf[2 * NQ + li[f]];
+        acc = __builtin_fmaf(lh, fh, acc);
+        cor = __builtin_fmaf(lh, fl, __builtin_fmaf(ll, fh, cor));
+      }
+      L[0] = (double)acc + (double)cor;  // the whole lift term
+      L[1] = L[2] = L[3] = L[4] = L[5] = 0.0;
+    } else if constexpr (FACT_LIFT) {
       // l_-(normal index) for -y,-x,-z; l_+ for +x,+y,+z; lfac = [l_-, l_+]
       L[0] = lfac[j];
       L[1] = lfac[NQ + i];
@@ -396,12 +462,17 @@ __device__ __forceinline__ void pencil_element(const StageParams &P, const doubl
       }
     }
     // tensorprod_Lift_hexahedral order (mod_dg_optr_kernel.f90:277-283)
-    double lift = L[0] * f0;
-    lift = madd<FMA>(L[1], f1, lift);
-    lift = madd<FMA>(L[2], f2, lift);
-    lift = madd<FMA>(L[3], f3, lift);
-    lift = madd<FMA>(L[4], flux4, lift);
-    lift = madd<FMA>(L[5], flux5, lift);
+    double lift;
+    if constexpr (ORDER == kMixed) {
+      lift = L[0];
+    } else {
+      lift = L[0] * f0;
+      lift = madd<FMA>(L[1], f1, lift);
+      lift = madd<FMA>(L[2], f2, lift);
+      lift = madd<FMA>(L[3], f3, lift);
+      lift = madd<FMA>(L[4], flux4, lift);
+      lift = madd<FMA>(L[5], flux5, lift);
+    }
     // dqdt = -(Es1 Dx + Es2 Dy + Es3 Dz + Lift) (mod_advect3d_eq.f90:194-198)
     double sum;
     if constexpr (LEAN) {

@@ -387,3 +387,36 @@ def test_state_errors(gpu):
     with pytest.raises(ValueError):
         s.set_fields(*(np.ones((27, 7), order="F") for _ in range(4)))
     s.close()
+
+
+@pytest.mark.parametrize("case", [c for c in CS.RUN_CASES if c[1] + 1 in (2, 4, 8)],
+                         ids=lambda c: CS.case_key(*c[0], c[1], c[2]))
+def test_mixed_precision_error(gpu, case):
+    """DGK_FLAG_MIXED (FP32 split contractions and lift, FP64 flux
+    differences, residual and RK): its error against the oracle is that of
+    FP32 arithmetic with split operands, not zero and not larger.  Measured
+    about 1e-9 rel-L2 after 1 step and 6e-9 after 100 steps on these
+    meshes; the bound here is 5e-8 after nstep, 1e-8 after one step."""
+    ne, p, nstep, vel, kind = case
+    _, D, L = dgk.read_operator(p)
+    om = O.Mesh(*ne, p)
+    q, u, v, w, dt = CS.run_inputs(ne, p, vel, kind, om.pos_en)
+    for steps, tol in ((1, 1e-8), (nstep, 5e-8)):
+        say(f"[mixed] {ne} p={p} steps={steps}")
+        F = oracle_fields(om, q, u, v, w)
+        om.run(*F, dt, steps, O.OPT1)
+        s = dgk.Solver.structured(*ne, p, D, L, flags=dgk.FLAG_MIXED)
+        s.set_fields(q, u, v, w)
+        s.run(dt, steps)
+        err = rel_l2(s.get_field(), F[0][:, :om.Ne])
+        assert 1e-13 < err <= tol, err
+        s.close()
+
+
+def test_mixed_precision_refused_where_not_built(gpu):
+    for p, flags in ((5, dgk.FLAG_MIXED), (7, dgk.FLAG_MIXED | dgk.FLAG_FMA),
+                     (7, dgk.FLAG_MIXED | dgk.FLAG_NODE_KERNEL)):
+        _, D, L = dgk.read_operator(p)
+        with pytest.raises(dgk.DGKError) as e:
+            dgk.Solver.structured(2, 2, 2, p, D, L, flags=flags)
+        assert e.value.code == dgk.ERR_ARG
