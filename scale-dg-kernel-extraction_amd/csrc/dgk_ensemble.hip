@@ -55,16 +55,11 @@ struct EnsArgs {
   const EnsInst *inst;
   const EnsOps *ops;
   const int2 *items;     // (instance, first element), grouped by class
-  const double *qin;
-  double *qout, *q0;
-  const double *u, *v, *w;
-  const int32_t *nbr;
-  double a, b;
-  int mode;
   int ncls;
   int cls_nq[kEnsMaxClass];
   int cls_block[kEnsMaxClass + 1];  // first workgroup of each class (multiples of 8)
   int cls_item[kEnsMaxClass + 1];   // first item of each class
+  const StageParams *params;        // per instance, for this launch's stage and parity
 };
 
 bool ens_supported(int nq) { return nq == 2 || nq == 4 || nq == 6 || nq == 8 || nq == 12; }
@@ -88,83 +83,43 @@ size_t ens_item_lds(int nq, bool cv) {
   }
 }
 
-// The operator tables of an item are read from memory the compiler cannot
-// prove constant, so their addresses arrive in VGPRs; the scalar row loads
-// of the pencil sweeps need them in SGPRs (they are uniform by construction).
-template <class T>
-__device__ __forceinline__ T *uniform_ptr(T *p) {
-  const unsigned long long v = (unsigned long long)p;
-  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
-  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
-  return (T *)(((unsigned long long)hi << 32) | lo);
-}
-
-// *p for a uniform address, returned in SGPRs: the tables are read with
-// vector loads (the compiler cannot prove them constant) and every dword is
-// made scalar, which keeps the instance constants out of the VGPR budget.
-template <class T>
-__device__ __forceinline__ T uniform_load(const T *p) {
-  static_assert(sizeof(T) % 4 == 0, "dword-sized records");
-  constexpr int N = sizeof(T) / 4;
-  const unsigned *w = reinterpret_cast<const unsigned *>(p);
-  unsigned v[N];
-#pragma unroll
-  for (int t = 0; t < N; ++t) v[t] = __builtin_amdgcn_readfirstlane(w[t]);
-  return __builtin_bit_cast(T, v);
-}
-
-// One item: the pencil stage (lean LDS layout) of the elements first ..
-// first+EPB-1 of an instance; lanes beyond EPB*Nq^2 only join the barriers.
-template <int NQ, int ORDER, bool FMA, bool FACT, bool CV>
-__device__ __forceinline__ void ens_pencil(const StageParams &P, const EnsOps &O, long long first,
-                                           double *smem) {
-  constexpr int NFP = NQ * NQ, EPB = kEnsThreads / NFP;
-  constexpr int PER = PencilShape<NQ>::lean_elem(CV);
-  const int tid = threadIdx.x;
-  const bool act = tid < EPB * NFP;
-  const int el = act ? tid / NFP : 0;
-  const int pl = act ? tid - el * NFP : 0;
-  long long e = first + el;
-  if (!act || e >= P.Ne) e = -1;
-  pencil_element<NQ, ORDER, FMA, FACT, true, CV, true>(P, uniform_ptr(O.Drow), O.LiftP, O.lfac,
-                                                        smem + el * PER, pl, e, first, act);
-}
+// The per-instance parameter blocks, operator slots and work items are read
+// through the constant address space: scalar loads at each use, which the
+// compiler may repeat instead of holding the values in registers across the
+// whole stage.
+typedef __attribute__((address_space(4))) const StageParams CParams;
+typedef __attribute__((address_space(4))) const EnsInst CInst;
+typedef __attribute__((address_space(4))) const EnsOps COps;
+typedef __attribute__((address_space(4))) const int CInt;
 
 // The items [first, first+count) of one class on nb workgroups (lb = this
-// workgroup's index among them; lb % 8 is its XCD).
+// workgroup's index among them; lb % 8 is its XCD).  An item is the lean
+// pencil stage of the elements first .. first+EPB-1 of an instance; lanes
+// beyond EPB*Nq^2 only join the barriers.
 template <int NQ, int ORDER, bool FMA, bool FACT, bool CV>
 __device__ __forceinline__ void ens_items(const EnsArgs &A, int first, int count, int lb, int nb,
                                           double *smem) {
+  constexpr int NFP = NQ * NQ, EPB = kEnsThreads / NFP;
+  constexpr int PER = PencilShape<NQ>::lean_elem(CV);
   const int xcd = lb % kNumXcd, slot = lb / kNumXcd, nslot = nb / kNumXcd;
   const int per = (count + kNumXcd - 1) / kNumXcd;
   const int end = first + min(count, (xcd + 1) * per);
   for (int it = first + xcd * per + slot; it < end; it += nslot) {
-    const int2 item = uniform_load(A.items + it);
-    const EnsInst I = uniform_load(A.inst + item.x);
-    const EnsOps O = uniform_load(A.ops + I.op);
-    StageParams P{};
-    P.qin = A.qin + I.off;
-    P.qout = A.qout + I.off;
-    P.q0 = A.q0 + I.off;
-    P.u = A.u + I.off;
-    P.v = A.v + I.off;
-    P.w = A.w + I.off;
-    P.D = O.D;
-    P.Lift = O.Lift;
-    P.nbr = A.nbr + I.nbr_off;
-#pragma unroll
-    for (int f = 0; f < 6; ++f) P.fs[f] = I.fs[f];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      P.es[d] = I.es[d];
-      P.cvel[d] = I.cvel[d];
-    }
-    P.Ne = I.ne;
-    P.a = A.a;
-    P.b = A.b;
-    P.dt = I.dt;
-    P.mode = A.mode;
-    ens_pencil<NQ, ORDER, FMA, FACT, CV>(P, O, item.y, smem);
+    const int inst = ((CInt *)A.items)[2 * it], first_el = ((CInt *)A.items)[2 * it + 1];
+    CParams &P = ((CParams *)A.params)[inst];
+    COps &O = ((COps *)A.ops)[((CInst *)A.inst)[inst].op];
+    // the lane's line and element, made opaque per item so that the address
+    // arithmetic derived from them is not hoisted out of the loop and held
+    // in registers through every item
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    const bool act = tid < EPB * NFP;
+    const int el = act ? tid / NFP : 0;
+    const int pl = act ? tid - el * NFP : 0;
+    long long e = first_el + el;
+    if (!act || e >= P.Ne) e = -1;
+    pencil_element<NQ, ORDER, FMA, FACT, true, CV, true, CParams>(
+        P, O.Drow, O.LiftP, O.lfac, smem + el * PER, pl, e, first_el, act);
     __syncthreads();  // LDS is reused by the next item
   }
 }
@@ -217,6 +172,10 @@ struct dgk_ensemble {
   int32_t *nbr = nullptr;
   EnsInst *dinst = nullptr;
   EnsOps *dops = nullptr;
+  std::vector<EnsOps> hops;    // host copy
+  // stage parameter blocks: [cur parity][stage][instance], rebuilt when dt or
+  // a uniform velocity changes
+  StageParams *dparams = nullptr;
   int2 *ditems = nullptr;
   int nitems = 0;
   // classes of equal order and the split of the grid between them
@@ -247,21 +206,47 @@ void ens_drop_graphs(dgk_ensemble *s) {
     }
 }
 
+// The stage parameter blocks of every instance for the 2 parities x 3
+// stages: parity c reads q[c] and writes q[c^1] (the ping-pong of the state).
+int ens_upload_params(dgk_ensemble *s) {
+  std::vector<StageParams> h((size_t)6 * s->n);
+  for (int c = 0; c < 2; ++c)
+    for (int st = 0; st < 3; ++st)
+      for (int i = 0; i < s->n; ++i) {
+        const EnsInst &I = s->inst[i];
+        const EnsOps &O = s->hops[I.op];
+        StageParams P{};
+        P.qin = s->q[c] + I.off;
+        P.qout = s->q[c ^ 1] + I.off;
+        P.q0 = s->q0 + I.off;
+        P.u = s->u + I.off;
+        P.v = s->v + I.off;
+        P.w = s->w + I.off;
+        P.D = O.D;
+        P.Lift = O.Lift;
+        P.nbr = s->nbr + I.nbr_off;
+        for (int f = 0; f < 6; ++f) P.fs[f] = I.fs[f];
+        for (int d = 0; d < 3; ++d) {
+          P.es[d] = I.es[d];
+          P.cvel[d] = I.cvel[d];
+        }
+        P.Ne = I.ne;
+        P.a = kRkA[st];
+        P.b = kRkB[st];
+        P.dt = I.dt;
+        P.mode = st == 0 ? kModeStage1 : kModeStageN;
+        h[((size_t)c * 3 + st) * s->n + i] = P;
+      }
+  HIPOK(hipMemcpy(s->dparams, h.data(), h.size() * sizeof(StageParams), hipMemcpyHostToDevice));
+  return DGK_OK;
+}
+
 EnsArgs ens_args(const dgk_ensemble *s, int stage) {
   EnsArgs A{};
   A.inst = s->dinst;
   A.ops = s->dops;
   A.items = s->ditems;
-  A.qin = s->q[s->cur];
-  A.qout = s->q[s->cur ^ 1];
-  A.q0 = s->q0;
-  A.u = s->u;
-  A.v = s->v;
-  A.w = s->w;
-  A.nbr = s->nbr;
-  A.a = kRkA[stage];
-  A.b = kRkB[stage];
-  A.mode = stage == 0 ? kModeStage1 : kModeStageN;
+  A.params = s->dparams + ((size_t)s->cur * 3 + stage) * s->n;
   A.ncls = s->ncls;
   for (int c = 0; c < s->ncls; ++c) A.cls_nq[c] = s->cls_nq[c];
   for (int c = 0; c <= s->ncls; ++c) {
@@ -338,7 +323,8 @@ extern "C" {
 int dgk_ensemble_destroy(dgk_ensemble *s) {
   if (!s) return DGK_OK;
   ens_drop_graphs(s);
-  void *bufs[] = {s->q[0], s->q[1], s->q0, s->u, s->v, s->w, s->nbr, s->dinst, s->dops, s->ditems};
+  void *bufs[] = {s->q[0], s->q[1], s->q0, s->u, s->v, s->w, s->nbr, s->dinst, s->dops, s->ditems,
+                  s->dparams};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   for (double *b : s->opbufs)
@@ -379,7 +365,7 @@ int dgk_ensemble_create(dgk_ensemble **out, int n, const int32_t *p, const int32
   }
   // operators, one slot per distinct order
   std::map<int, int> slot;
-  std::vector<EnsOps> hops;
+  std::vector<EnsOps> &hops = s->hops;
   for (int i = 0; i < n && !rc; ++i) {
     if (slot.count(p[i])) continue;
     const int Nq = p[i] + 1, Np = Nq * Nq * Nq, Nfp = Nq * Nq;
@@ -478,6 +464,7 @@ int dgk_ensemble_create(dgk_ensemble **out, int n, const int32_t *p, const int32
   if (!rc) rc = ens_alloc(&s->dinst, (size_t)n);
   if (!rc) rc = ens_alloc(&s->dops, hops.size());
   if (!rc) rc = ens_alloc(&s->ditems, items.size());
+  if (!rc) rc = ens_alloc(&s->dparams, (size_t)6 * n);
   if (!rc && (hipMemcpy(s->nbr, nbr.data(), nbr.size() * sizeof(int32_t), hipMemcpyHostToDevice) ||
               hipMemcpy(s->dops, hops.data(), hops.size() * sizeof(EnsOps), hipMemcpyHostToDevice) ||
               hipMemcpy(s->ditems, items.data(), items.size() * sizeof(int2),
@@ -486,6 +473,7 @@ int dgk_ensemble_create(dgk_ensemble **out, int n, const int32_t *p, const int32
     rc = fail(DGK_ERR_HIP, "ensemble table upload failed");
   if (!rc && hipMemset(s->q0, 0, off * sizeof(double)) != hipSuccess)
     rc = fail(DGK_ERR_HIP, "hipMemset failed");
+  if (!rc) rc = ens_upload_params(s);
   if (!rc) {
     s->fn = pick_ens((flags & DGK_FLAG_FMA) != 0, (flags & DGK_FLAG_FACTORED_LIFT) != 0,
                      (flags & DGK_FLAG_CONST_VELOCITY) != 0);
@@ -522,6 +510,8 @@ int dgk_ensemble_set_fields(dgk_ensemble *s, int i, const double *q, const doubl
       I.cvel[d] = vel[d][0];
     }
     HIPOK(hipMemcpy(s->dinst + i, &I, sizeof(EnsInst), hipMemcpyHostToDevice));
+    int rc = ens_upload_params(s);
+    if (rc) return rc;
   }
   double *dst[4] = {s->q[s->cur] + I.off, s->u + I.off, s->v + I.off, s->w + I.off};
   const double *src[4] = {q, u, v, w};
@@ -545,7 +535,7 @@ int dgk_ensemble_set_dt(dgk_ensemble *s, const double *dt) {
   HIPOK(hipStreamSynchronize(s->stream));
   for (int i = 0; i < s->n; ++i) s->inst[i].dt = dt[i];
   HIPOK(hipMemcpy(s->dinst, s->inst.data(), s->n * sizeof(EnsInst), hipMemcpyHostToDevice));
-  return DGK_OK;
+  return ens_upload_params(s);
 }
 
 int dgk_ensemble_step(dgk_ensemble *s) {

@@ -230,8 +230,11 @@ __device__ __forceinline__ void st(double *p, double v) {
 // LEAN: the x phase runs first and leaves Es1*D_x in a buffer S that the y
 // phase turns into Es1*D_x + Es2*D_y (the reference's summation order), so
 // that with uniform velocity two volume buffers suffice instead of three.
-template <int NQ, int ORDER, bool FMA, bool FACT_LIFT, bool NT, bool CV, bool LEAN = false>
-__device__ __forceinline__ void pencil_element(const StageParams &P, const double *__restrict__ Drow,
+// PT: the parameter block's type, StageParams or a StageParams in the
+// constant address space (read on demand with scalar loads).
+template <int NQ, int ORDER, bool FMA, bool FACT_LIFT, bool NT, bool CV, bool LEAN = false,
+          class PT = const StageParams>
+__device__ __forceinline__ void pencil_element(PT &P, const double *__restrict__ Drow,
                                                const double *__restrict__ Lift,
                                                const double *__restrict__ lfac, double *blk,
                                                int pl, long long e, long long e_any,
