@@ -106,3 +106,24 @@ def test_create_rejects_bad_tables_before_device_use():
                                m["normal_fn"], m["Escale"], m["Fscale"], m["halo_src_map"],
                                typeid=7)
     assert e.value.code == dgk.ERR_OPTTYPE
+
+
+def test_ensemble_workload_draws():
+    """The 64-instance workload is a pure function of its seed: orders from
+    {3,5,7,11}, per-axis counts from {8,12,16,24}, velocity components in
+    [-1,1], widths in [0.05,0.2], dt at CFL 0.1 of each instance."""
+    from dgk import ic
+    a = ic.ensemble_workload(64, 1004)
+    b = ic.ensemble_workload(64, 1004)
+    assert a == b and len(a) == 64
+    assert a != ic.ensemble_workload(64, 1005)
+    for s in a:
+        assert s["p"] in (3, 5, 7, 11)
+        assert all(n in (8, 12, 16, 24) for n in s["ne"])
+        assert all(-1.0 <= c <= 1.0 for c in s["vel"])
+        assert all(0.0 <= c < 1.0 for c in s["centre"])
+        assert 0.05 <= s["width"] <= 0.2
+        h = [1.0 / n for n in s["ne"]]
+        assert abs(s["dt"] * (2 * s["p"] + 1) * sum(abs(u) / d for u, d in zip(s["vel"], h))
+                   - 0.1) < 1e-15
+    assert {s["p"] for s in a} == {3, 5, 7, 11}

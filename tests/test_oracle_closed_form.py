@@ -174,3 +174,4 @@ def test_halo_slots_copy_their_sources():
     d = d - np.round(d)  # periodic unit cube
     assert np.max(np.abs(d)) < 1e-14
     assert halo.sum() == m.NhaloNode
+
