@@ -220,56 +220,34 @@ __device__ __forceinline__ void st(double *p, double v) {
 // CV: constant velocity (P.cvel): only q and q0 stream through HBM; the
 // products q*u etc. and the face velocities are those of uniform u,v,w
 // fields, so the results are the same bits.
-// One element's lines (or EPW elements of one wavefront) of the pencil stage.
-// e is this lane's element or -1 (idle lane), e_any any element of the same
-// workgroup (for the address math of idle lanes), blk the LDS block of this
-// lane's element, pl the lane's line index a + NQ*b.  act = false marks a
-// lane beyond the workgroup's elements: it writes no LDS (blk may then be
-// any block).  Every lane of the workgroup must call it (it holds two
-// workgroup barriers, three when LEAN).
-// LEAN: the x phase runs first and leaves Es1*D_x in a buffer S that the y
-// phase turns into Es1*D_x + Es2*D_y (the reference's summation order), so
-// that with uniform velocity two volume buffers suffice instead of three.
-// PT: the parameter block's type, StageParams or a StageParams in the
-// constant address space (read on demand with scalar loads).
-template <int NQ, int ORDER, bool FMA, bool FACT_LIFT, bool NT, bool CV, bool LEAN = false,
-          class PT = const StageParams>
-__device__ __forceinline__ void pencil_element(PT &P, const double *__restrict__ Drow,
-                                               const double *__restrict__ Lift,
-                                               const double *__restrict__ lfac, double *blk,
-                                               int pl, long long e, long long e_any,
-                                               bool act = true) {
-  // Lift: the pair-packed copy described at the lift loads below
-  using S = PencilShape<NQ>;
-  constexpr int NP = S::NP, NFP = S::NFP, PAD = S::PAD, FPAD = S::FPAD;
-  constexpr int NVOL = (LEAN && CV) ? 2 : 3;
-  const int la = pl % NQ, lb = pl / NQ;
-  double *Bq = blk;
-  double *Bu = Bq + PAD;  // u, then D_x (LEAN: u, then the partial sum S)
-  double *Bv = Bu + PAD;  // v, then D_y
-  double *BS = Bu;        // LEAN: Es1 D_x, then + Es2 D_y
-  double *Bf = Bq + NVOL * PAD;  // fluxes of faces -y, +y, -x, +x
-  const bool valid = e >= 0;
-  const long long eb = valid ? e : e_any;  // a valid element for address math
-
-  // ---- loads: own z-line (i,j) = (la,lb) and six neighbour face traces ----
-  const long long base = eb * NP + pl;
+// The global loads of one lane of the pencil stage: its own z-line
+// (i,j) = (la,lb) of q, u, v, w (and q0) and the six neighbour face traces.
+template <int NQ>
+struct PencilLoads {
   double q[NQ], u[NQ], v[NQ], w[NQ], q0[NQ];
+  double qP0, qP1, qP2, qP3, qP4, qP5, vP0, uP1, vP2, uP3, wP4, wP5;
+};
+
+// Q0: also load q0 (stages 2 and 3 need it).  eb is a valid element.
+template <int NQ, bool CV, bool Q0, class PT>
+__device__ __forceinline__ void pencil_load(PT &P, long long eb, int pl, PencilLoads<NQ> &L) {
+  constexpr int NP = NQ * NQ * NQ, NFP = NQ * NQ;
+  const int la = pl % NQ, lb = pl / NQ;
+  const long long base = eb * NP + pl;
   const bool need_q0 = P.mode == kModeStageN;
 #pragma unroll
   for (int k = 0; k < NQ; ++k) {
-    q[k] = P.qin[base + NFP * k];
+    L.q[k] = P.qin[base + NFP * k];
     if constexpr (CV) {
-      u[k] = P.cvel[0];
-      v[k] = P.cvel[1];
-      w[k] = P.cvel[2];
+      L.u[k] = P.cvel[0];
+      L.v[k] = P.cvel[1];
+      L.w[k] = P.cvel[2];
     } else {
-      u[k] = P.u[base + NFP * k];
-      v[k] = P.v[base + NFP * k];
-      w[k] = P.w[base + NFP * k];
+      L.u[k] = P.u[base + NFP * k];
+      L.v[k] = P.v[base + NFP * k];
+      L.w[k] = P.w[base + NFP * k];
     }
-    // LEAN reads q0 in the final phase
-    if constexpr (!LEAN) q0[k] = need_q0 ? P.q0[base + NFP * k] : 0.0;
+    if constexpr (Q0) L.q0[k] = need_q0 ? P.q0[base + NFP * k] : 0.0;
   }
   const int32_t *nb = P.nbr + eb * 6;
   // z faces at (i,j): -z (face 4) sees the neighbour's k = NQ-1, +z (face 5)
@@ -282,12 +260,53 @@ __device__ __forceinline__ void pencil_element(PT &P, const double *__restrict__
   // x faces at (j,k) = (la,lb): -x (face 3) sees i = NQ-1, +x (face 1) i = 0
   const long long a3 = (long long)nb[3] * NP + (NQ - 1) + NQ * pl;
   const long long a1 = (long long)nb[1] * NP + NQ * pl;
-  const double qP4 = P.qin[a4], qP5 = P.qin[a5];
-  const double qP0 = P.qin[a0], qP2 = P.qin[a2];
-  const double qP3 = P.qin[a3], qP1 = P.qin[a1];
-  const double wP4 = CV ? P.cvel[2] : P.w[a4], wP5 = CV ? P.cvel[2] : P.w[a5];
-  const double vP0 = CV ? P.cvel[1] : P.v[a0], vP2 = CV ? P.cvel[1] : P.v[a2];
-  const double uP3 = CV ? P.cvel[0] : P.u[a3], uP1 = CV ? P.cvel[0] : P.u[a1];
+  L.qP4 = P.qin[a4];
+  L.qP5 = P.qin[a5];
+  L.qP0 = P.qin[a0];
+  L.qP2 = P.qin[a2];
+  L.qP3 = P.qin[a3];
+  L.qP1 = P.qin[a1];
+  L.wP4 = CV ? P.cvel[2] : P.w[a4];
+  L.wP5 = CV ? P.cvel[2] : P.w[a5];
+  L.vP0 = CV ? P.cvel[1] : P.v[a0];
+  L.vP2 = CV ? P.cvel[1] : P.v[a2];
+  L.uP3 = CV ? P.cvel[0] : P.u[a3];
+  L.uP1 = CV ? P.cvel[0] : P.u[a1];
+}
+
+// The stage of one element from its loaded lines.  LATE_Q0: q0 is not in L
+// and is read here, at the start (non-LEAN) or in the final phase (LEAN).
+template <int NQ, int ORDER, bool FMA, bool FACT_LIFT, bool NT, bool CV, bool LEAN, bool LATE_Q0,
+          class PT>
+__device__ __forceinline__ void pencil_compute(PT &P, const double *__restrict__ Drow,
+                                               const double *__restrict__ Lift,
+                                               const double *__restrict__ lfac, double *blk,
+                                               int pl, long long e, long long eb, bool act,
+                                               PencilLoads<NQ> &L) {
+  // Lift: the pair-packed copy described at the lift loads below
+  using S = PencilShape<NQ>;
+  constexpr int NP = S::NP, NFP = S::NFP, PAD = S::PAD, FPAD = S::FPAD;
+  constexpr int NVOL = (LEAN && CV) ? 2 : 3;
+  const int la = pl % NQ, lb = pl / NQ;
+  double *Bq = blk;
+  double *Bu = Bq + PAD;  // u, then D_x (LEAN: u, then the partial sum S)
+  double *Bv = Bu + PAD;  // v, then D_y
+  double *BS = Bu;        // LEAN: Es1 D_x, then + Es2 D_y
+  double *Bf = Bq + NVOL * PAD;  // fluxes of faces -y, +y, -x, +x
+  const bool valid = e >= 0;
+  const long long base = eb * NP + pl;
+  const bool need_q0 = P.mode == kModeStageN;
+  double (&q)[NQ] = L.q;
+  double (&u)[NQ] = L.u;
+  double (&v)[NQ] = L.v;
+  double (&w)[NQ] = L.w;
+  double (&q0)[NQ] = L.q0;
+  if constexpr (LATE_Q0 && !LEAN) {
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) q0[k] = need_q0 ? P.q0[base + NFP * k] : 0.0;
+  }
+  const double qP4 = L.qP4, qP5 = L.qP5, qP0 = L.qP0, qP2 = L.qP2, qP3 = L.qP3, qP1 = L.qP1;
+  const double wP4 = L.wP4, wP5 = L.wP5, vP0 = L.vP0, vP2 = L.vP2, uP3 = L.uP3, uP1 = L.uP1;
 
   // ---- z phase (registers; LEAN: in the final phase) ---------------------
   // LEAN keeps neither q, q0 nor D_z in registers through the x and y
@@ -499,6 +518,33 @@ __device__ __forceinline__ void pencil_element(PT &P, const double *__restrict__
       st<NT>(P.qout + idx, qn);
     }
   }
+}
+
+// One element's lines (or EPW elements of one wavefront) of the pencil stage.
+// e is this lane's element or -1 (idle lane), e_any any element of the same
+// workgroup (for the address math of idle lanes), blk the LDS block of this
+// lane's element, pl the lane's line index a + NQ*b.  act = false marks a
+// lane beyond the workgroup's elements: it writes no LDS (blk may then be
+// any block).  Every lane of the workgroup must call it (it holds two
+// workgroup barriers, three when LEAN).
+// LEAN: the x phase runs first and leaves Es1*D_x in a buffer S that the y
+// phase turns into Es1*D_x + Es2*D_y (the reference's summation order), so
+// that with uniform velocity two volume buffers suffice instead of three.
+// PT: the parameter block's type, StageParams or a StageParams in the
+// constant address space (read on demand with scalar loads).
+template <int NQ, int ORDER, bool FMA, bool FACT_LIFT, bool NT, bool CV, bool LEAN = false,
+          class PT = const StageParams>
+__device__ __forceinline__ void pencil_element(PT &P, const double *__restrict__ Drow,
+                                               const double *__restrict__ Lift,
+                                               const double *__restrict__ lfac, double *blk,
+                                               int pl, long long e, long long e_any,
+                                               bool act = true) {
+  const long long eb = e >= 0 ? e : e_any;  // a valid element for address math
+  PencilLoads<NQ> L;
+  // LEAN reads q0 in the final phase
+  pencil_load<NQ, CV, !LEAN>(P, eb, pl, L);
+  pencil_compute<NQ, ORDER, FMA, FACT_LIFT, NT, CV, LEAN, false>(P, Drow, Lift, lfac, blk, pl, e,
+                                                                 eb, act, L);
 }
 
 template <int NQ, int ORDER, bool FMA, bool FACT_LIFT, bool NT, bool CV>
