@@ -194,6 +194,7 @@ struct dgk_ensemble {
   int grid = 0;
   hipGraphExec_t graph[2] = {nullptr, nullptr};
   bool fields_set = false;
+  bool dt_set = false;
 };
 
 namespace {
@@ -349,6 +350,10 @@ int dgk_ensemble_create(dgk_ensemble **out, int n, const int32_t *p, const int32
     int32_t sz[7];
     int rc = dgk_mesh_sizes(nex[i], ney[i], nez[i], p[i], sz);
     if (rc) return rc;
+    for (int d = 0; dom && d < 3; ++d)
+      if (!(dom[3 * i + d] > 0.0) || dom[3 * i + d] > 1e300)
+        return fail(DGK_ERR_ARG, "instance %d: domain length %g is not positive and finite", i,
+                    dom[3 * i + d]);
     if (!ens_supported(p[i] + 1))
       return fail(DGK_ERR_ORDER, "instance %d: the ensemble runs PolyOrder 1, 3, 5, 7 or 11 "
                                  "(got %d)", i, p[i]);
@@ -499,6 +504,7 @@ int dgk_ensemble_create(dgk_ensemble **out, int n, const int32_t *p, const int32
 int dgk_ensemble_set_fields(dgk_ensemble *s, int i, const double *q, const double *u,
                             const double *v, const double *w) {
   if (!s || i < 0 || i >= s->n || !q || !u || !v || !w) return fail(DGK_ERR_ARG, "bad argument");
+  HIPOK(hipStreamSynchronize(s->stream));  // steps in flight still read the fields
   EnsInst &I = s->inst[i];
   const size_t cnt = (size_t)I.ne * s->np[i];
   if (s->flags & DGK_FLAG_CONST_VELOCITY) {
@@ -533,7 +539,11 @@ int dgk_ensemble_get_field(dgk_ensemble *s, int i, double *q) {
 int dgk_ensemble_set_dt(dgk_ensemble *s, const double *dt) {
   if (!s || !dt) return fail(DGK_ERR_ARG, "bad argument");
   HIPOK(hipStreamSynchronize(s->stream));
+  for (int i = 0; i < s->n; ++i)
+    if (!(dt[i] >= 0.0) || dt[i] > 1e300)
+      return fail(DGK_ERR_ARG, "instance %d: dt %g is not a finite non-negative number", i, dt[i]);
   for (int i = 0; i < s->n; ++i) s->inst[i].dt = dt[i];
+  s->dt_set = true;
   HIPOK(hipMemcpy(s->dinst, s->inst.data(), s->n * sizeof(EnsInst), hipMemcpyHostToDevice));
   return ens_upload_params(s);
 }
@@ -541,6 +551,7 @@ int dgk_ensemble_set_dt(dgk_ensemble *s, const double *dt) {
 int dgk_ensemble_step(dgk_ensemble *s) {
   if (!s) return fail(DGK_ERR_ARG, "null ensemble");
   if (!s->fields_set) return fail(DGK_ERR_STATE, "no fields set");
+  if (!s->dt_set) return fail(DGK_ERR_STATE, "no time steps set (dgk_ensemble_set_dt)");
   const int c0 = s->cur;
   if (!s->graph[c0]) {
     hipGraph_t g;
