@@ -217,16 +217,23 @@ StageKernel make_pencil() {
   return k;
 }
 
-// Elements per workgroup of the group pencil kernel: Nq 2, 4, 8 (variant
-// bit kPencilWide only) one wavefront, as pencil_kernel but in the lean
-// layout; Nq 6 narrow 7 elements (four wavefronts) or wide 16 (nine); Nq 12
-// narrow 1 (three) or wide 2 (five); Nq 16 one.
+// Elements per workgroup of the group pencil kernel, narrow / wide (the
+// workgroup is the lanes rounded up to whole wavefronts): Nq 2, 4, 8 (wide
+// only) one wavefront, as pencil_kernel but in the lean layout; Nq 3 28 / 64;
+// Nq 5 10 / 23; Nq 6 7 / 16; Nq 7 5 / 11; Nq 9 2 / 3; Nq 10 2 / 3; Nq 11 1
+// / 2; Nq 12 1 / 2; Nq 13..16 one shape, 1 element.
 int group_epb(int Nq, bool wide) {
   switch (Nq) {
+  case 3: return wide ? 64 : 28;
+  case 5: return wide ? 23 : 10;
   case 6: return wide ? 16 : 7;
+  case 7: return wide ? 11 : 5;
+  case 9: return wide ? 3 : 2;
+  case 10: return wide ? 3 : 2;
+  case 11: return wide ? 2 : 1;
   case 12: return wide ? 2 : 1;
-  case 16: return 1;
-  default: return 64 / (Nq * Nq);
+  case 2: case 4: case 8: return 64 / (Nq * Nq);
+  default: return 1;
   }
 }
 
@@ -259,15 +266,14 @@ StageKernel pick_pencil_nq(int order, bool fma, bool fact, bool nt, bool cv) {
              : pick_pencil_fact<NQ, kOpt1, false>(fact, nt, cv);
 }
 
-bool pencil_nq(int Nq) {
-  return Nq == 2 || Nq == 4 || Nq == 6 || Nq == 8 || Nq == 12 || Nq == 16;
-}
+bool pencil_nq(int Nq) { return Nq >= 2 && Nq <= 16; }
 
 // Nq 2, 4, 8: pencil_kernel, or with `wide` the group kernel (one
 // wavefront, lean layout); Nq 6, 12, 16: the group kernel.
 StageKernel pick_pencil(int Nq, int order, bool fma, bool fact, bool nt, bool cv, bool wide) {
-  if (order == kMixed && (Nq == 6 || Nq == 12 || Nq == 16 || wide)) return StageKernel{};
-  if (Nq == 6 || Nq == 12 || Nq == 16 || wide)
+  const bool one_wave = Nq == 2 || Nq == 4 || Nq == 8;
+  if (order == kMixed && (!one_wave || wide)) return StageKernel{};
+  if (!one_wave || wide)
     return pick_pencil_group(Nq, order, fma, fact, cv, group_epb(Nq, wide));
   switch (Nq) {
   case 2: return pick_pencil_nq<2>(order, fma, fact, nt, cv);
@@ -389,10 +395,10 @@ int default_pencil_variant(int Nq) {
 }
 
 // Tuning variants of the solver's stage kernel: store flavour x element
-// order x workgroup shape (Nq 16 has one shape).
+// order x workgroup shape (Nq 13-16 have one shape).
 int pencil_variants(const dgk_solver *s) {
   if (!s->sk.pencil) return 1;
-  return (s->Nq == 16 || s->order == kMixed) ? kPencilWide : kPencilVariants;
+  return (s->Nq >= 13 || s->order == kMixed) ? kPencilWide : kPencilVariants;
 }
 
 int configure_launch(dgk_solver *s) {
@@ -403,8 +409,8 @@ int configure_launch(dgk_solver *s) {
     return fail(DGK_ERR_ARG, "DGK_FLAG_MIXED needs the one-wavefront pencil kernel (Nq 2, 4 "
                              "or 8, compressed connectivity) without DGK_FLAG_FMA");
   if ((s->flags & DGK_FLAG_CONST_VELOCITY) && !use_pencil)
-    return fail(DGK_ERR_ARG, "DGK_FLAG_CONST_VELOCITY needs the pencil kernel (Nq 2, 4, 6, 8, "
-                             "12 or 16, compressed connectivity)");
+    return fail(DGK_ERR_ARG, "DGK_FLAG_CONST_VELOCITY needs the pencil kernel (compressed "
+                             "connectivity, no DGK_FLAG_NODE_KERNEL)");
   if (use_pencil)
     s->sk = pick_pencil(s->Nq, s->order, s->fma, (s->flags & DGK_FLAG_FACTORED_LIFT) != 0,
                         (var & kPencilNtStore) != 0, (s->flags & DGK_FLAG_CONST_VELOCITY) != 0,
@@ -416,8 +422,8 @@ int configure_launch(dgk_solver *s) {
                              "or 8, compressed connectivity) without DGK_FLAG_FMA");
   if (!s->sk.kernel) return fail(DGK_ERR_ORDER, "Nq %d not supported", s->Nq);
   if ((s->flags & DGK_FLAG_FACTORED_LIFT) && !use_pencil)
-    return fail(DGK_ERR_ARG, "DGK_FLAG_FACTORED_LIFT needs the pencil kernel (Nq 2, 4, 6, 8, "
-                             "12 or 16, compressed connectivity)");
+    return fail(DGK_ERR_ARG, "DGK_FLAG_FACTORED_LIFT needs the pencil kernel (compressed "
+                             "connectivity, no DGK_FLAG_NODE_KERNEL)");
   HIPOK(hipFuncSetAttribute(s->sk.kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)s->sk.lds));
   const long long ngroups = (s->Ne + s->sk.epb - 1) / s->sk.epb;

@@ -1,16 +1,18 @@
-// The multi-wavefront pencil stage (pencil_group_kernel) for every pencil
-// order, built in a translation unit of its own.
+// Instantiation of the multi-wavefront pencil stage (pencil_group_kernel) for
+// one Nq: shared by the translation units that build the orders in parallel.
+#pragma once
+
 #include "dgk_launch.hpp"
 #include "dgk_pencil.hpp"
 
 namespace dgk {
-namespace {
+namespace group_impl {
 
-// Workgroup limit per Nq: Nq 12 and 16 need about 150-230 VGPRs, so their
+// Workgroup limit per Nq: Nq 9-12 and 13-16 need about 150-230 VGPRs, so their
 // workgroups stay at five and four wavefronts (at most two per SIMD, which
-// allows 256 VGPRs); the other orders allow nine.
+// allows 256 VGPRs); lower orders allow nine.
 template <int NQ>
-constexpr int group_maxth() { return NQ == 16 ? 256 : (NQ == 12 ? 320 : 576); }
+constexpr int group_maxth() { return NQ >= 13 ? 256 : (NQ >= 9 ? 320 : 576); }
 
 template <int NQ, int ORDER, bool FMA, bool FACT, bool CV>
 void launch_group(const StageKernel &k, const StageParams &P, const HostOps &H, const double *,
@@ -51,18 +53,6 @@ StageKernel pick_nq(int order, bool fma, bool fact, bool cv, int epb) {
              : pick_fact<NQ, kOpt1, false>(fact, cv, epb);
 }
 
-}  // namespace
 
-StageKernel pick_pencil_group(int Nq, int order, bool fma, bool fact, bool cv, int epb) {
-  switch (Nq) {
-  case 2: return pick_nq<2>(order, fma, fact, cv, epb);
-  case 4: return pick_nq<4>(order, fma, fact, cv, epb);
-  case 6: return pick_nq<6>(order, fma, fact, cv, epb);
-  case 8: return pick_nq<8>(order, fma, fact, cv, epb);
-  case 12: return pick_nq<12>(order, fma, fact, cv, epb);
-  case 16: return pick_nq<16>(order, fma, fact, cv, epb);
-  default: return StageKernel{};
-  }
-}
-
+}  // namespace group_impl
 }  // namespace dgk

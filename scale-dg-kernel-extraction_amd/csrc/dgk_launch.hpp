@@ -34,8 +34,12 @@ struct StageKernel {
 };
 
 // The multi-wavefront pencil stage (pencil_group_kernel, lean LDS layout)
-// for Nq in {2, 4, 6, 8, 12, 16} with epb elements per workgroup; an empty
-// descriptor if epb does not fit the kernel's thread limit.
+// for Nq in 2..16 with epb elements per workgroup; an empty descriptor if
+// epb does not fit the kernel's thread limit.
 StageKernel pick_pencil_group(int Nq, int order, bool fma, bool fact, bool cv, int epb);
+// the same for Nq 2-8, 9-12 and 13-16 (separate translation units)
+StageKernel pick_pencil_group_low(int Nq, int order, bool fma, bool fact, bool cv, int epb);
+StageKernel pick_pencil_group_mid(int Nq, int order, bool fma, bool fact, bool cv, int epb);
+StageKernel pick_pencil_group_high(int Nq, int order, bool fma, bool fact, bool cv, int epb);
 
 }  // namespace dgk

@@ -30,8 +30,7 @@ namespace dgk {
 
 template <int NQ>
 struct PencilShape {
-  static_assert(NQ == 2 || NQ == 4 || NQ == 6 || NQ == 8 || NQ == 12 || NQ == 16,
-                "pencil stage: Nq in {2, 4, 6, 8, 12, 16}");
+  static_assert(NQ >= 2 && NQ <= 16, "pencil stage: Nq in 2..16");
   static constexpr int NP = NQ * NQ * NQ;
   static constexpr int NFP = NQ * NQ;
   static constexpr int EPW = NFP <= 64 ? 64 / NFP : 1;  // elements per wavefront (Nq 2, 4, 8)
@@ -57,79 +56,262 @@ constexpr size_t pencil_group_lds(int epb, bool cv) {
 template <int NQ>
 __device__ __forceinline__ int pidx(int n) { return n + n / NQ; }
 
+// Scalar loads of NDW consecutive dwords (a row of D: 2 Nq dwords) into
+// SGPRs, in one statement with its wait: the widest loads first, each
+// destination early-clobber so that no load overwrites the base address a
+// later one still reads.  `after` orders the statement after a VGPR value.
+template <int NDW>
+__device__ __forceinline__ void sgpr_load(const double *p, double after, int (&dw)[NDW]);
+
+template <>
+__device__ __forceinline__ void sgpr_load<4>(const double *p, double after, int (&dw)[4]) {
+  typedef int t0 __attribute__((ext_vector_type(4)));
+  t0 x0;
+  asm volatile("s_load_dwordx4 %0, %1, 0x0\n\ts_waitcnt lgkmcnt(0)"
+               : "=s"(x0)
+               : "s"(p), "v"(after));
+#pragma unroll
+  for (int t = 0; t < 4; ++t) dw[0 + t] = x0[t];
+}
+
+template <>
+__device__ __forceinline__ void sgpr_load<6>(const double *p, double after, int (&dw)[6]) {
+  typedef int t0 __attribute__((ext_vector_type(4)));
+  t0 x0;
+  typedef int t1 __attribute__((ext_vector_type(2)));
+  t1 x1;
+  asm volatile("s_load_dwordx4 %0, %2, 0x0\n\ts_load_dwordx2 %1, %2, 0x10\n\ts_waitcnt lgkmcnt(0)"
+               : "=&s"(x0), "=&s"(x1)
+               : "s"(p), "v"(after));
+#pragma unroll
+  for (int t = 0; t < 4; ++t) dw[0 + t] = x0[t];
+#pragma unroll
+  for (int t = 0; t < 2; ++t) dw[4 + t] = x1[t];
+}
+
+template <>
+__device__ __forceinline__ void sgpr_load<8>(const double *p, double after, int (&dw)[8]) {
+  typedef int t0 __attribute__((ext_vector_type(8)));
+  t0 x0;
+  asm volatile("s_load_dwordx8 %0, %1, 0x0\n\ts_waitcnt lgkmcnt(0)"
+               : "=s"(x0)
+               : "s"(p), "v"(after));
+#pragma unroll
+  for (int t = 0; t < 8; ++t) dw[0 + t] = x0[t];
+}
+
+template <>
+__device__ __forceinline__ void sgpr_load<10>(const double *p, double after, int (&dw)[10]) {
+  typedef int t0 __attribute__((ext_vector_type(8)));
+  t0 x0;
+  typedef int t1 __attribute__((ext_vector_type(2)));
+  t1 x1;
+  asm volatile("s_load_dwordx8 %0, %2, 0x0\n\ts_load_dwordx2 %1, %2, 0x20\n\ts_waitcnt lgkmcnt(0)"
+               : "=&s"(x0), "=&s"(x1)
+               : "s"(p), "v"(after));
+#pragma unroll
+  for (int t = 0; t < 8; ++t) dw[0 + t] = x0[t];
+#pragma unroll
+  for (int t = 0; t < 2; ++t) dw[8 + t] = x1[t];
+}
+
+template <>
+__device__ __forceinline__ void sgpr_load<12>(const double *p, double after, int (&dw)[12]) {
+  typedef int t0 __attribute__((ext_vector_type(8)));
+  t0 x0;
+  typedef int t1 __attribute__((ext_vector_type(4)));
+  t1 x1;
+  asm volatile("s_load_dwordx8 %0, %2, 0x0\n\ts_load_dwordx4 %1, %2, 0x20\n\ts_waitcnt lgkmcnt(0)"
+               : "=&s"(x0), "=&s"(x1)
+               : "s"(p), "v"(after));
+#pragma unroll
+  for (int t = 0; t < 8; ++t) dw[0 + t] = x0[t];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) dw[8 + t] = x1[t];
+}
+
+template <>
+__device__ __forceinline__ void sgpr_load<14>(const double *p, double after, int (&dw)[14]) {
+  typedef int t0 __attribute__((ext_vector_type(8)));
+  t0 x0;
+  typedef int t1 __attribute__((ext_vector_type(4)));
+  t1 x1;
+  typedef int t2 __attribute__((ext_vector_type(2)));
+  t2 x2;
+  asm volatile("s_load_dwordx8 %0, %3, 0x0\n\ts_load_dwordx4 %1, %3, 0x20\n\ts_load_dwordx2 %2, %3, 0x30\n\ts_waitcnt lgkmcnt(0)"
+               : "=&s"(x0), "=&s"(x1), "=&s"(x2)
+               : "s"(p), "v"(after));
+#pragma unroll
+  for (int t = 0; t < 8; ++t) dw[0 + t] = x0[t];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) dw[8 + t] = x1[t];
+#pragma unroll
+  for (int t = 0; t < 2; ++t) dw[12 + t] = x2[t];
+}
+
+template <>
+__device__ __forceinline__ void sgpr_load<16>(const double *p, double after, int (&dw)[16]) {
+  typedef int t0 __attribute__((ext_vector_type(16)));
+  t0 x0;
+  asm volatile("s_load_dwordx16 %0, %1, 0x0\n\ts_waitcnt lgkmcnt(0)"
+               : "=s"(x0)
+               : "s"(p), "v"(after));
+#pragma unroll
+  for (int t = 0; t < 16; ++t) dw[0 + t] = x0[t];
+}
+
+template <>
+__device__ __forceinline__ void sgpr_load<18>(const double *p, double after, int (&dw)[18]) {
+  typedef int t0 __attribute__((ext_vector_type(16)));
+  t0 x0;
+  typedef int t1 __attribute__((ext_vector_type(2)));
+  t1 x1;
+  asm volatile("s_load_dwordx16 %0, %2, 0x0\n\ts_load_dwordx2 %1, %2, 0x40\n\ts_waitcnt lgkmcnt(0)"
+               : "=&s"(x0), "=&s"(x1)
+               : "s"(p), "v"(after));
+#pragma unroll
+  for (int t = 0; t < 16; ++t) dw[0 + t] = x0[t];
+#pragma unroll
+  for (int t = 0; t < 2; ++t) dw[16 + t] = x1[t];
+}
+
+template <>
+__device__ __forceinline__ void sgpr_load<20>(const double *p, double after, int (&dw)[20]) {
+  typedef int t0 __attribute__((ext_vector_type(16)));
+  t0 x0;
+  typedef int t1 __attribute__((ext_vector_type(4)));
+  t1 x1;
+  asm volatile("s_load_dwordx16 %0, %2, 0x0\n\ts_load_dwordx4 %1, %2, 0x40\n\ts_waitcnt lgkmcnt(0)"
+               : "=&s"(x0), "=&s"(x1)
+               : "s"(p), "v"(after));
+#pragma unroll
+  for (int t = 0; t < 16; ++t) dw[0 + t] = x0[t];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) dw[16 + t] = x1[t];
+}
+
+template <>
+__device__ __forceinline__ void sgpr_load<22>(const double *p, double after, int (&dw)[22]) {
+  typedef int t0 __attribute__((ext_vector_type(16)));
+  t0 x0;
+  typedef int t1 __attribute__((ext_vector_type(4)));
+  t1 x1;
+  typedef int t2 __attribute__((ext_vector_type(2)));
+  t2 x2;
+  asm volatile("s_load_dwordx16 %0, %3, 0x0\n\ts_load_dwordx4 %1, %3, 0x40\n\ts_load_dwordx2 %2, %3, 0x50\n\ts_waitcnt lgkmcnt(0)"
+               : "=&s"(x0), "=&s"(x1), "=&s"(x2)
+               : "s"(p), "v"(after));
+#pragma unroll
+  for (int t = 0; t < 16; ++t) dw[0 + t] = x0[t];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) dw[16 + t] = x1[t];
+#pragma unroll
+  for (int t = 0; t < 2; ++t) dw[20 + t] = x2[t];
+}
+
+template <>
+__device__ __forceinline__ void sgpr_load<24>(const double *p, double after, int (&dw)[24]) {
+  typedef int t0 __attribute__((ext_vector_type(16)));
+  t0 x0;
+  typedef int t1 __attribute__((ext_vector_type(8)));
+  t1 x1;
+  asm volatile("s_load_dwordx16 %0, %2, 0x0\n\ts_load_dwordx8 %1, %2, 0x40\n\ts_waitcnt lgkmcnt(0)"
+               : "=&s"(x0), "=&s"(x1)
+               : "s"(p), "v"(after));
+#pragma unroll
+  for (int t = 0; t < 16; ++t) dw[0 + t] = x0[t];
+#pragma unroll
+  for (int t = 0; t < 8; ++t) dw[16 + t] = x1[t];
+}
+
+template <>
+__device__ __forceinline__ void sgpr_load<26>(const double *p, double after, int (&dw)[26]) {
+  typedef int t0 __attribute__((ext_vector_type(16)));
+  t0 x0;
+  typedef int t1 __attribute__((ext_vector_type(8)));
+  t1 x1;
+  typedef int t2 __attribute__((ext_vector_type(2)));
+  t2 x2;
+  asm volatile("s_load_dwordx16 %0, %3, 0x0\n\ts_load_dwordx8 %1, %3, 0x40\n\ts_load_dwordx2 %2, %3, 0x60\n\ts_waitcnt lgkmcnt(0)"
+               : "=&s"(x0), "=&s"(x1), "=&s"(x2)
+               : "s"(p), "v"(after));
+#pragma unroll
+  for (int t = 0; t < 16; ++t) dw[0 + t] = x0[t];
+#pragma unroll
+  for (int t = 0; t < 8; ++t) dw[16 + t] = x1[t];
+#pragma unroll
+  for (int t = 0; t < 2; ++t) dw[24 + t] = x2[t];
+}
+
+template <>
+__device__ __forceinline__ void sgpr_load<28>(const double *p, double after, int (&dw)[28]) {
+  typedef int t0 __attribute__((ext_vector_type(16)));
+  t0 x0;
+  typedef int t1 __attribute__((ext_vector_type(8)));
+  t1 x1;
+  typedef int t2 __attribute__((ext_vector_type(4)));
+  t2 x2;
+  asm volatile("s_load_dwordx16 %0, %3, 0x0\n\ts_load_dwordx8 %1, %3, 0x40\n\ts_load_dwordx4 %2, %3, 0x60\n\ts_waitcnt lgkmcnt(0)"
+               : "=&s"(x0), "=&s"(x1), "=&s"(x2)
+               : "s"(p), "v"(after));
+#pragma unroll
+  for (int t = 0; t < 16; ++t) dw[0 + t] = x0[t];
+#pragma unroll
+  for (int t = 0; t < 8; ++t) dw[16 + t] = x1[t];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) dw[24 + t] = x2[t];
+}
+
+template <>
+__device__ __forceinline__ void sgpr_load<30>(const double *p, double after, int (&dw)[30]) {
+  typedef int t0 __attribute__((ext_vector_type(16)));
+  t0 x0;
+  typedef int t1 __attribute__((ext_vector_type(8)));
+  t1 x1;
+  typedef int t2 __attribute__((ext_vector_type(4)));
+  t2 x2;
+  typedef int t3 __attribute__((ext_vector_type(2)));
+  t3 x3;
+  asm volatile("s_load_dwordx16 %0, %4, 0x0\n\ts_load_dwordx8 %1, %4, 0x40\n\ts_load_dwordx4 %2, %4, 0x60\n\ts_load_dwordx2 %3, %4, 0x70\n\ts_waitcnt lgkmcnt(0)"
+               : "=&s"(x0), "=&s"(x1), "=&s"(x2), "=&s"(x3)
+               : "s"(p), "v"(after));
+#pragma unroll
+  for (int t = 0; t < 16; ++t) dw[0 + t] = x0[t];
+#pragma unroll
+  for (int t = 0; t < 8; ++t) dw[16 + t] = x1[t];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) dw[24 + t] = x2[t];
+#pragma unroll
+  for (int t = 0; t < 2; ++t) dw[28 + t] = x3[t];
+}
+
+template <>
+__device__ __forceinline__ void sgpr_load<32>(const double *p, double after, int (&dw)[32]) {
+  typedef int t0 __attribute__((ext_vector_type(16)));
+  t0 x0;
+  typedef int t1 __attribute__((ext_vector_type(16)));
+  t1 x1;
+  asm volatile("s_load_dwordx16 %0, %2, 0x0\n\ts_load_dwordx16 %1, %2, 0x40\n\ts_waitcnt lgkmcnt(0)"
+               : "=&s"(x0), "=&s"(x1)
+               : "s"(p), "v"(after));
+#pragma unroll
+  for (int t = 0; t < 16; ++t) dw[0 + t] = x0[t];
+#pragma unroll
+  for (int t = 0; t < 16; ++t) dw[16 + t] = x1[t];
+}
+
 // Row k of the 1D operator, D(k,0..NQ-1), from the row-major copy Drow into
-// SGPRs with one scalar load.  The load is written out (with its wait in the
-// same statement) because left to itself the compiler loads all NQ^2
-// coefficients at the top of the kernel and spills them to VGPR lanes, which
-// costs more VALU issue slots than the sweeps themselves.
-// `after` is a value the row must not be loaded before (the previous
-// output): it keeps one row of SGPRs live at a time.  Rows longer than 16
-// dwords take two loads; their destinations are early-clobber so that the
-// first cannot overwrite the base address the second still reads.
+// SGPRs.  The loads are written out (with their wait in the same statement)
+// because left to itself the compiler loads all NQ^2 coefficients at the top
+// of the kernel and spills them to VGPR lanes, which costs more VALU issue
+// slots than the sweeps themselves.  `after` is a value the row must not be
+// loaded before (the previous output): it keeps one row of SGPRs live at a
+// time.
 template <int NQ>
 __device__ __forceinline__ void sgpr_row(const double *Drow, int k, double after,
                                          double (&r)[NQ]) {
-  const double *p = Drow + k * NQ;
-  constexpr int NDW = 2 * NQ;  // dwords per row
-  typedef int v16 __attribute__((ext_vector_type(16)));
-  typedef int v8 __attribute__((ext_vector_type(8)));
-  typedef int v4 __attribute__((ext_vector_type(4)));
-  int dw[NDW];
-  if constexpr (NQ == 16) {
-    v16 x, y;
-    asm volatile("s_load_dwordx16 %0, %2, 0x0\n\ts_load_dwordx16 %1, %2, 0x40\n\t"
-                 "s_waitcnt lgkmcnt(0)"
-                 : "=&s"(x), "=&s"(y)
-                 : "s"(p), "v"(after));
-#pragma unroll
-    for (int t = 0; t < 16; ++t) {
-      dw[t] = x[t];
-      dw[16 + t] = y[t];
-    }
-  } else if constexpr (NQ == 12) {
-    v16 x;
-    v8 y;
-    asm volatile("s_load_dwordx16 %0, %2, 0x0\n\ts_load_dwordx8 %1, %2, 0x40\n\t"
-                 "s_waitcnt lgkmcnt(0)"
-                 : "=&s"(x), "=&s"(y)
-                 : "s"(p), "v"(after));
-#pragma unroll
-    for (int t = 0; t < 16; ++t) dw[t] = x[t];
-#pragma unroll
-    for (int t = 0; t < 8; ++t) dw[16 + t] = y[t];
-  } else if constexpr (NQ == 8) {
-    v16 x;
-    asm volatile("s_load_dwordx16 %0, %1, 0x0\n\ts_waitcnt lgkmcnt(0)"
-                 : "=s"(x)
-                 : "s"(p), "v"(after));
-#pragma unroll
-    for (int t = 0; t < 16; ++t) dw[t] = x[t];
-  } else if constexpr (NQ == 6) {
-    v8 x;
-    v4 y;
-    asm volatile("s_load_dwordx8 %0, %2, 0x0\n\ts_load_dwordx4 %1, %2, 0x20\n\t"
-                 "s_waitcnt lgkmcnt(0)"
-                 : "=&s"(x), "=&s"(y)
-                 : "s"(p), "v"(after));
-#pragma unroll
-    for (int t = 0; t < 8; ++t) dw[t] = x[t];
-#pragma unroll
-    for (int t = 0; t < 4; ++t) dw[8 + t] = y[t];
-  } else if constexpr (NQ == 4) {
-    v8 x;
-    asm volatile("s_load_dwordx8 %0, %1, 0x0\n\ts_waitcnt lgkmcnt(0)"
-                 : "=s"(x)
-                 : "s"(p), "v"(after));
-#pragma unroll
-    for (int t = 0; t < 8; ++t) dw[t] = x[t];
-  } else {
-    v4 x;
-    asm volatile("s_load_dwordx4 %0, %1, 0x0\n\ts_waitcnt lgkmcnt(0)"
-                 : "=s"(x)
-                 : "s"(p), "v"(after));
-#pragma unroll
-    for (int t = 0; t < 4; ++t) dw[t] = x[t];
-  }
+  int dw[2 * NQ];
+  sgpr_load<2 * NQ>(Drow + k * NQ, after, dw);
 #pragma unroll
   for (int l = 0; l < NQ; ++l)
     r[l] = __builtin_bit_cast(double, (unsigned long long)(unsigned)dw[2 * l] |
@@ -205,7 +387,8 @@ __device__ __forceinline__ void sweep(const double *Drow, const double (&f)[NQ],
 //                       structured meshes (a launch parameter, not a
 //                       template argument)
 //   kPencilWide         Nq 2, 4, 8: the group kernel's lean layout (one
-//                       wavefront); Nq 6, 12: more elements per workgroup
+//                       wavefront); Nq 3, 5, 6, 7, 9-12: more elements per
+//                       workgroup (Nq 13-16 have one shape)
 constexpr int kPencilNtStore = 1;
 constexpr int kPencilLinearOrder = 2;
 constexpr int kPencilWide = 4;

@@ -118,16 +118,16 @@ def test_batched_directional_gradlike_var5(gpu, golden, p):
 # tendency (advect3d_eq_cal_tend)
 
 
-# orders with a pencil stage kernel (Nq 2, 4, 8 one wavefront; 6, 12, 16
-# several)
-PENCIL_NQ = (2, 4, 6, 8, 12, 16)
+# orders with a pencil stage kernel (every Nq; 2, 4, 8 on one wavefront, the
+# others on multi-wavefront workgroups)
+PENCIL_NQ = tuple(range(2, 17))
 
 def _solver_variants(ne, p, D, L, typeid, flags=0):
     m = dgk.mesh_setup(*ne, p)
     s = dgk.Solver.structured(*ne, p, D, L, typeid=typeid, flags=flags)
     assert s.kernel_kind == ("pencil" if p + 1 in PENCIL_NQ else "node")
     yield "structured-" + s.kernel_kind, s
-    if p + 1 in (2, 4, 8):
+    if p + 1 in PENCIL_NQ:
         s = dgk.Solver.structured(*ne, p, D, L, typeid=typeid, flags=flags | dgk.FLAG_NODE_KERNEL)
         assert s.kernel_kind == "node"
         yield "structured-node", s
@@ -294,7 +294,7 @@ def test_stretched_mesh_keeps_tables(gpu):
     s.close()
 
 
-@pytest.mark.parametrize("p", [1, 3, 5, 7, 11, 15])
+@pytest.mark.parametrize("p", range(1, 16))
 @pytest.mark.parametrize("ne", [(3, 2, 9), (4, 5, 3)])
 def test_every_tuning_variant_is_bit_exact(gpu, p, ne):
     """All code shapes the autotuner may pick (tiled and linear element
@@ -319,7 +319,7 @@ def test_every_tuning_variant_is_bit_exact(gpu, p, ne):
     s.close()
 
 
-@pytest.mark.parametrize("p", [1, 3, 5, 7, 11, 15])
+@pytest.mark.parametrize("p", [1, 2, 3, 4, 5, 7, 8, 9, 11, 13, 15])
 @pytest.mark.parametrize("factored", [False, True])
 def test_constant_velocity_form(gpu, p, factored):
     """DGK_FLAG_CONST_VELOCITY streams no velocity and gives the same bits as
