@@ -284,7 +284,7 @@ def main():
     s = dgk.Solver.structured(*ne, p, D, L, flags=VARIANTS[args.variant])
     if s.kernel_kind != "pencil":
         kernel_name = "stage_kernel"
-    elif p + 1 in (6, 12, 16) or s.variant & 4:
+    elif p + 1 not in (2, 4, 8) or s.variant & 4:
         kernel_name = "pencil_group_kernel"
     else:
         kernel_name = "pencil_kernel"

@@ -48,7 +48,8 @@ extern "C" {
 #define DGK_FLAG_FORCE_TABLES 0x2  /* always use the per-node VMapM/VMapP,
                                       normal, Fscale and Escale tables       */
 #define DGK_FLAG_NODE_KERNEL 0x4   /* thread-per-node stage kernel even where
-                                      the pencil kernel applies (Nq 2, 4, 8) */
+                                      the pencil kernels apply (compressed
+                                      connectivity) */
 #define DGK_FLAG_FACTORED_LIFT 0x8 /* lift with the 1D vectors
                                       Lift(:,1,1,4) and Lift(:,1,1,2) (exact
                                       to ~1e-14 relative, not bit-identical) */
