@@ -38,9 +38,9 @@ struct PencilShape {
   static constexpr int FPAD = NFP + NFP / NQ;  // padded face buffer
   static constexpr int PER_ELEM = 3 * PAD + 4 * FPAD;
   static constexpr size_t LDS = sizeof(double) * EPW * PER_ELEM;
-  // LDS of one element in the lean layout: q, the partial sum S and (for
-  // non-uniform velocity) u and v share three volume buffers, or two.
-  static constexpr int lean_elem(bool cv) { return (cv ? 2 : 3) * PAD + 4 * FPAD; }
+  // LDS of one element in the lean layout: two volume buffers (q and the
+  // partial sum S) and the four x/y face-flux buffers.
+  static constexpr int LEAN_ELEM = 2 * PAD + 4 * FPAD;
 };
 
 // Threads and LDS of a multi-wavefront pencil workgroup of epb elements:
@@ -49,8 +49,8 @@ struct PencilShape {
 template <int NQ>
 constexpr int pencil_group_threads(int epb) { return (epb * NQ * NQ + 63) / 64 * 64; }
 template <int NQ>
-constexpr size_t pencil_group_lds(int epb, bool cv) {
-  return sizeof(double) * epb * PencilShape<NQ>::lean_elem(cv);
+constexpr size_t pencil_group_lds(int epb) {
+  return sizeof(double) * epb * PencilShape<NQ>::LEAN_ELEM;
 }
 
 template <int NQ>
@@ -411,8 +411,10 @@ struct PencilLoads {
   double qP0, qP1, qP2, qP3, qP4, qP5, vP0, uP1, vP2, uP3, wP4, wP5;
 };
 
-// Q0: also load q0 (stages 2 and 3 need it).  eb is a valid element.
-template <int NQ, bool CV, bool Q0, class PT>
+// Q0: also load q0 (stages 2 and 3 need it).  UV: also load the z-lines of
+// u and v (the lean layout reads their x- and y-lines itself).  eb is a
+// valid element.
+template <int NQ, bool CV, bool Q0, bool UV, class PT>
 __device__ __forceinline__ void pencil_load(PT &P, long long eb, int pl, PencilLoads<NQ> &L) {
   constexpr int NP = NQ * NQ * NQ, NFP = NQ * NQ;
   const int la = pl % NQ, lb = pl / NQ;
@@ -426,8 +428,10 @@ __device__ __forceinline__ void pencil_load(PT &P, long long eb, int pl, PencilL
       L.v[k] = P.cvel[1];
       L.w[k] = P.cvel[2];
     } else {
-      L.u[k] = P.u[base + NFP * k];
-      L.v[k] = P.v[base + NFP * k];
+      if constexpr (UV) {
+        L.u[k] = P.u[base + NFP * k];
+        L.v[k] = P.v[base + NFP * k];
+      }
       L.w[k] = P.w[base + NFP * k];
     }
     if constexpr (Q0) L.q0[k] = need_q0 ? P.q0[base + NFP * k] : 0.0;
@@ -469,15 +473,16 @@ __device__ __forceinline__ void pencil_compute(PT &P, const double *__restrict__
   // Lift: the pair-packed copy described at the lift loads below
   using S = PencilShape<NQ>;
   constexpr int NP = S::NP, NFP = S::NFP, PAD = S::PAD, FPAD = S::FPAD;
-  constexpr int NVOL = (LEAN && CV) ? 2 : 3;
+  constexpr int NVOL = LEAN ? 2 : 3;
   const int la = pl % NQ, lb = pl / NQ;
   double *Bq = blk;
-  double *Bu = Bq + PAD;  // u, then D_x (LEAN: u, then the partial sum S)
+  double *Bu = Bq + PAD;  // u, then D_x (LEAN: the partial sum S)
   double *Bv = Bu + PAD;  // v, then D_y
   double *BS = Bu;        // LEAN: Es1 D_x, then + Es2 D_y
   double *Bf = Bq + NVOL * PAD;  // fluxes of faces -y, +y, -x, +x
   const bool valid = e >= 0;
-  const long long base = eb * NP + pl;
+  const long long ebase = eb * NP;
+  const long long base = ebase + pl;
   const bool need_q0 = P.mode == kModeStageN;
   double (&q)[NQ] = L.q;
   double (&u)[NQ] = L.u;
@@ -510,7 +515,7 @@ __device__ __forceinline__ void pencil_compute(PT &P, const double *__restrict__
     for (int k = 0; k < NQ; ++k) {
       const int s = pidx<NQ>(pl + NFP * k);
       Bq[s] = q[k];
-      if constexpr (!CV) {
+      if constexpr (!CV && !LEAN) {
         Bu[s] = u[k];
         Bv[s] = v[k];
       }
@@ -521,13 +526,14 @@ __device__ __forceinline__ void pencil_compute(PT &P, const double *__restrict__
 
   if constexpr (LEAN) {
     // ---- x phase: line (j,k) = (la,lb); S = Es1 D_x ------------------------
+    // The x-line of u is read from memory (Nq consecutive values per lane).
     {
       double qx[NQ], ux[NQ], fx[NQ], dx[NQ];
 #pragma unroll
+      for (int l = 0; l < NQ; ++l) ux[l] = CV ? P.cvel[0] : P.u[ebase + l + NQ * pl];
+#pragma unroll
       for (int l = 0; l < NQ; ++l) {
-        const int s = pidx<NQ>(l + NQ * pl);
-        qx[l] = Bq[s];
-        ux[l] = CV ? P.cvel[0] : Bu[s];
+        qx[l] = Bq[pidx<NQ>(l + NQ * pl)];
         fx[l] = qx[l] * ux[l];
       }
       sweep<NQ, ORDER, FMA, true>(Drow, fx, dx);
@@ -542,13 +548,14 @@ __device__ __forceinline__ void pencil_compute(PT &P, const double *__restrict__
     }
     __syncthreads();
     // ---- y phase: line (i,k) = (la,lb); S += Es2 D_y -----------------------
+    // The y-line of v is read from memory.
     {
       double qy[NQ], vy[NQ], fy[NQ], dy[NQ];
 #pragma unroll
+      for (int l = 0; l < NQ; ++l) vy[l] = CV ? P.cvel[1] : P.v[ebase + la + NQ * l + NFP * lb];
+#pragma unroll
       for (int l = 0; l < NQ; ++l) {
-        const int s = pidx<NQ>(la + NQ * l + NFP * lb);
-        qy[l] = Bq[s];
-        vy[l] = CV ? P.cvel[1] : Bv[s];
+        qy[l] = Bq[pidx<NQ>(la + NQ * l + NFP * lb)];
         fy[l] = qy[l] * vy[l];
       }
       sweep<NQ, ORDER, FMA, false>(Drow, fy, dy);
@@ -711,8 +718,9 @@ __device__ __forceinline__ void pencil_compute(PT &P, const double *__restrict__
 // any block).  Every lane of the workgroup must call it (it holds two
 // workgroup barriers, three when LEAN).
 // LEAN: the x phase runs first and leaves Es1*D_x in a buffer S that the y
-// phase turns into Es1*D_x + Es2*D_y (the reference's summation order), so
-// that with uniform velocity two volume buffers suffice instead of three.
+// phase turns into Es1*D_x + Es2*D_y (the reference's summation order); the
+// x- and y-lines of u and v come straight from memory, so two volume buffers
+// (q and S) suffice instead of three, whatever the velocity.
 // PT: the parameter block's type, StageParams or a StageParams in the
 // constant address space (read on demand with scalar loads).
 template <int NQ, int ORDER, bool FMA, bool FACT_LIFT, bool NT, bool CV, bool LEAN = false,
@@ -725,7 +733,7 @@ __device__ __forceinline__ void pencil_element(PT &P, const double *__restrict__
   const long long eb = e >= 0 ? e : e_any;  // a valid element for address math
   PencilLoads<NQ> L;
   // LEAN reads q0 in the final phase
-  pencil_load<NQ, CV, !LEAN>(P, eb, pl, L);
+  pencil_load<NQ, CV, !LEAN, !LEAN>(P, eb, pl, L);
   pencil_compute<NQ, ORDER, FMA, FACT_LIFT, NT, CV, LEAN, false>(P, Drow, Lift, lfac, blk, pl, e,
                                                                  eb, act, L);
 }
@@ -777,7 +785,7 @@ __global__ void __launch_bounds__(MAXTH)
                         const double *__restrict__ Lift, const double *__restrict__ lfac,
                         int epb) {
   constexpr int NFP = NQ * NQ;
-  constexpr int PER = PencilShape<NQ>::lean_elem(CV);
+  constexpr int PER = PencilShape<NQ>::LEAN_ELEM;
   extern __shared__ double smem[];
   const int tid = threadIdx.x;
   const bool act = tid < epb * NFP;
