@@ -7,8 +7,9 @@ the state resident in HBM.  The metric is GDOF-updates per second per RK
 stage: Ne*Np*3*steps / seconds.
 
 Prints one JSON line.  Beside the GPU figure it reports the stage kernel's
-HBM roofline fraction (algorithmic 48 B per DOF-update: read q,u,v,w and q0
-or write q0, write q) and a CPU baseline: the reference's own Fortran
+HBM roofline fraction (algorithmic bytes per DOF-update, averaged over the
+three stages: 40 B at stage 1 -- read q,u,v,w, write q -- and 48 B at stages
+2 and 3, which also read q0) and a CPU baseline: the reference's own Fortran
 modules (oracle/_ref, compiled from source) timed on the host cores over a
 bounded sample of the workload.
 
@@ -42,7 +43,12 @@ from dgk import ic  # noqa: E402
 
 METRIC = "GDOF-updates/s per RK stage at p=7, 64^3 elems; rel-L2 err vs FP64 oracle"
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_DOF = 48  # per DOF-update and stage, per-node FP64 velocity
+# Algorithmic HBM bytes per DOF-update, averaged over a step's three stages.
+# Stage 1 reads q,u,v,w and writes q (40 B): the state it reads becomes q0
+# without a copy.  Stages 2 and 3 also read q0 (48 B).
+BYTES_PER_DOF = (40 + 48 + 48) / 3
+# uniform velocity held as three scalars: 16 B at stage 1, 24 B after
+BYTES_PER_DOF_CV = (16 + 24 + 24) / 3
 PROFILES = os.path.join(REPO, "profiles")
 # solver variants: flags of dgk_create_structured
 VARIANTS = {
@@ -196,7 +202,7 @@ def single_workload(workload, variants, nstage=30):
         s.set_fields(q, *(np.full_like(q, c) for c in vel))
         var, _ = s.autotune(dt, nrep=6)
         ms = s.time_stages(dt, nstage)
-        bpd = 24 if VARIANTS[name] & dgk.FLAG_CONST_VELOCITY else BYTES_PER_DOF
+        bpd = BYTES_PER_DOF_CV if VARIANTS[name] & dgk.FLAG_CONST_VELOCITY else BYTES_PER_DOF
         res[name] = {"gdof_updates_per_s": round(s.Ne * s.Np / (ms * 1e-3) / 1e9, 2),
                      "kernel_us": round(ms * 1e3, 1), "variant": var,
                      "hbm_frac": round(bpd * s.Ne * s.Np / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
@@ -288,7 +294,8 @@ def main():
         kernel_name = "pencil_group_kernel"
     else:
         kernel_name = "pencil_kernel"
-    bytes_per_dof = 24 if VARIANTS[args.variant] & dgk.FLAG_CONST_VELOCITY else BYTES_PER_DOF
+    bytes_per_dof = (BYTES_PER_DOF_CV if VARIANTS[args.variant] & dgk.FLAG_CONST_VELOCITY
+                     else BYTES_PER_DOF)
     s.set_fields(q, u, v, w)
     w3 = np.einsum("i,j,k->ijk", *(lgl_weights(x1D),) * 3).ravel(order="F")
     J = (1.0 / ne[0]) * (1.0 / ne[1]) * (1.0 / ne[2]) / 8.0
@@ -313,7 +320,7 @@ def main():
     # solver's own stream
     nstage = max(30, 3 * args.steps)
     ms_stage = s.time_stages(dt, nstage)
-    bytes_launch = bytes_per_dof * dof
+    bytes_launch = round(bytes_per_dof * dof)  # mean over the three stages
     achieved = bytes_launch / (ms_stage * 1e-3) / 1e9
     qf = s.get_field()
     finite = bool(np.isfinite(qf).all())

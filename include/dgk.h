@@ -147,7 +147,8 @@ int dgk_set_fields(dgk_solver *s, const double *q, const double *u,
 /* Downloads one field (Np,NeA).  For q the halo slots hold the values of
  * the current owned q. */
 int dgk_get_field(dgk_solver *s, int which, double *out);
-/* Device address of a field (Np*NeA doubles) for zero-copy callers. */
+/* Device address of a field (Np*NeA doubles) for zero-copy callers; for q
+ * and q0 valid until the next stage or step. */
 int dgk_field_ptr(dgk_solver *s, int which, double **dptr);
 
 /* dqdt = advect3d_eq_cal_tend(q,u,v,w,...) on the device
@@ -155,7 +156,9 @@ int dgk_field_ptr(dgk_solver *s, int which, double **dptr);
 int dgk_cal_tend(dgk_solver *s);
 /* One SSP-RK3 stage (stage = 1..3): update_halo(q), tendency and
  * q = rk_a(stage) q0 + rk_b(stage) (q + dt dqdt) fused in one launch; stage
- * 1 also stores q0 <- q (main.f90:53-73). */
+ * 1 also makes q0 <- q (main.f90:53-73) by keeping the buffer it read as q0,
+ * so the device addresses of q and q0 (dgk_field_ptr) change with every
+ * stage. */
 int dgk_rk_stage(dgk_solver *s, int stage, double dt);
 /* One full time step (three stages) replayed from a captured hipGraph. */
 int dgk_step(dgk_solver *s, double dt);

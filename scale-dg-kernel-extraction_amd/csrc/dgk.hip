@@ -351,8 +351,11 @@ struct dgk_solver {
   int NeX = 0, NeY = 0, NeZ = 0;  // structured extents in the reference numbering (0: unknown)
   double cvel[3] = {0, 0, 0};     // uniform velocity (DGK_FLAG_CONST_VELOCITY)
   HostOps ops;
-  double *q[2] = {nullptr, nullptr};
-  double *q0 = nullptr, *u = nullptr, *v = nullptr, *w = nullptr, *dqdt = nullptr;
+  // State buffers: q and q0 rotate among three, so that no stage copies q
+  // to q0 (stage 1 takes the current buffer as q0; every stage writes the
+  // buffer that holds neither its input nor q0).
+  double *q[3] = {nullptr, nullptr, nullptr};
+  double *u = nullptr, *v = nullptr, *w = nullptr, *dqdt = nullptr;
   double *D = nullptr, *Lift = nullptr, *lfac = nullptr, *Drow = nullptr, *LiftP = nullptr;
   double *Drow32 = nullptr, *lfac32 = nullptr;  // FP32 split operators (mixed precision)
   int32_t *nbr = nullptr, *vmapM = nullptr, *vmapP = nullptr, *halo = nullptr;
@@ -361,10 +364,13 @@ struct dgk_solver {
   double *part = nullptr, *mm = nullptr;
   double *diag = nullptr;  // diagnostics: partials (8 x 1024), weights (Nq), result (8)
   int mm_cap = 0;
-  int cur = 0;
+  int cur = 0;  // buffer of q
+  int c0 = 2;   // buffer of q0 (never cur)
   hipStream_t stream = nullptr;
   StageKernel sk;
-  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  // one captured step per (cur, c0) it starts from, and the (cur, c0) it ends in
+  hipGraphExec_t graph[9] = {};
+  int graph_next[9][2] = {};
   double graph_dt = 0.0;
   bool fields_set = false;
 };
@@ -379,9 +385,13 @@ bool tiled_order(const dgk_solver *s) {
          !((s->variant >= 0 ? s->variant : 0) & kPencilLinearOrder);
 }
 
+// Tiles 32 elements wide: x-neighbours' face traces (a 64-byte-strided
+// gather that touches every line of the neighbour) are then read from L2 for
+// all but 1/32 of the elements; at 64^3, p=7 wider or narrower tiles cost
+// 2-7 % more.
 void tile_shape(const dgk_solver *s, int *tx, int *ty, long long *tseq) {
-  *tx = std::min(16, s->NeX);
-  *ty = std::min(8, s->NeY);
+  *tx = std::min(32, s->NeX);
+  *ty = std::min(16, s->NeY);
   const long long ntx = (s->NeX + *tx - 1) / *tx, nty = (s->NeY + *ty - 1) / *ty;
   const long long nk = (s->NeZ + kNumXcd - 1) / kNumXcd;
   *tseq = ntx * nty * (long long)(*tx) * (*ty) * nk;
@@ -455,11 +465,11 @@ int configure_launch(dgk_solver *s) {
 int alloc_fields(dgk_solver *s) {
   const size_t nf = (size_t)s->Np * (size_t)s->NeA;
   int rc;
-  if ((rc = dalloc(&s->q[0], nf)) || (rc = dalloc(&s->q[1], nf)) || (rc = dalloc(&s->q0, nf)) ||
+  if ((rc = dalloc(&s->q[0], nf)) || (rc = dalloc(&s->q[1], nf)) || (rc = dalloc(&s->q[2], nf)) ||
       (rc = dalloc(&s->u, nf)) || (rc = dalloc(&s->v, nf)) || (rc = dalloc(&s->w, nf)) ||
       (rc = dalloc(&s->dqdt, nf)))
     return rc;
-  HIPOK(hipMemsetAsync(s->q0, 0, nf * sizeof(double), s->stream));
+  HIPOK(hipMemsetAsync(s->q[2], 0, nf * sizeof(double), s->stream));
   HIPOK(hipMemsetAsync(s->dqdt, 0, nf * sizeof(double), s->stream));
   HIPOK(hipMemsetAsync(s->q[1], 0, nf * sizeof(double), s->stream));
   if ((rc = dalloc(&s->part, 2 * 1024))) return rc;
@@ -484,8 +494,8 @@ StageParams make_params(dgk_solver *s, int mode, double a, double b, double dt) 
     tile_shape(s, &P.ttx, &P.tty, &P.tseq);
   }
   P.qin = s->q[s->cur];
-  P.qout = s->q[s->cur ^ 1];
-  P.q0 = s->q0;
+  P.qout = s->q[3 - s->cur - s->c0];
+  P.q0 = s->q[mode == kModeStage1 ? s->cur : s->c0];
   P.u = s->u;
   P.v = s->v;
   P.w = s->w;
@@ -517,7 +527,11 @@ int launch_stage(dgk_solver *s, int mode, double a, double b, double dt) {
   StageParams P = make_params(s, mode, a, b, dt);
   s->sk.launch(s->sk, P, s->ops, s->Lift, s->stream);
   HIPOK(hipGetLastError());
-  if (mode != kModeTend) s->cur ^= 1;
+  if (mode != kModeTend) {
+    const int out = 3 - s->cur - s->c0;
+    if (mode == kModeStage1) s->c0 = s->cur;
+    s->cur = out;
+  }
   return DGK_OK;
 }
 
@@ -682,7 +696,7 @@ extern "C" {
 int dgk_destroy(dgk_solver *s) {
   if (!s) return DGK_OK;
   drop_graphs(s);
-  void *bufs[] = {s->q[0], s->q[1], s->q0, s->u, s->v, s->w, s->dqdt, s->D, s->Lift, s->lfac, s->Drow, s->LiftP,
+  void *bufs[] = {s->q[0], s->q[1], s->q[2], s->u, s->v, s->w, s->dqdt, s->D, s->Lift, s->lfac, s->Drow, s->LiftP,
                   s->nbr, s->vmapM, s->vmapP, s->halo, s->nrm, s->fscale, s->escale,
                   s->part, s->mm, s->diag, s->Drow32, s->lfac32};
   for (void *b : bufs)
@@ -840,6 +854,7 @@ int dgk_set_fields(dgk_solver *s, const double *q, const double *u, const double
     drop_graphs(s);  // the velocity is a launch argument
   }
   s->cur = 0;
+  s->c0 = 2;
   HIPOK(hipMemcpyAsync(s->q[0], q, nb, hipMemcpyHostToDevice, s->stream));
   HIPOK(hipMemcpyAsync(s->u, u, nb, hipMemcpyHostToDevice, s->stream));
   HIPOK(hipMemcpyAsync(s->v, v, nb, hipMemcpyHostToDevice, s->stream));
@@ -856,7 +871,7 @@ int dgk_field_ptr(dgk_solver *s, int which, double **dptr) {
   if (!s || !dptr) return fail(DGK_ERR_ARG, "null argument");
   switch (which) {
   case DGK_FIELD_Q: *dptr = s->q[s->cur]; break;
-  case DGK_FIELD_Q0: *dptr = s->q0; break;
+  case DGK_FIELD_Q0: *dptr = s->q[s->c0]; break;
   case DGK_FIELD_DQDT: *dptr = s->dqdt; break;
   case DGK_FIELD_U: *dptr = s->u; break;
   case DGK_FIELD_V: *dptr = s->v; break;
@@ -896,26 +911,30 @@ int dgk_step(dgk_solver *s, double dt) {
   if (!s) return fail(DGK_ERR_ARG, "null solver");
   if (!s->fields_set) return fail(DGK_ERR_STATE, "dgk_set_fields has not been called");
   if (s->graph_dt != dt) drop_graphs(s);
-  // Three stages flip the ping-pong buffers an odd number of times, so two
-  // graphs exist: one per starting buffer.
-  const int c0 = s->cur;
-  if (!s->graph[c0]) {
+  // The buffers a step reads and writes depend on where q and q0 sit when it
+  // starts: one graph per starting (cur, c0).
+  const int key = 3 * s->cur + s->c0, cur0 = s->cur, c00 = s->c0;
+  if (!s->graph[key]) {
     hipGraph_t g;
     HIPOK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
     int rc = DGK_OK;
     for (int st = 1; st <= 3 && !rc; ++st)
       rc = launch_stage(s, st == 1 ? kModeStage1 : kModeStageN, kRkA[st - 1], kRkB[st - 1], dt);
     hipError_t ce = hipStreamEndCapture(s->stream, &g);
-    s->cur = c0;
+    s->graph_next[key][0] = s->cur;
+    s->graph_next[key][1] = s->c0;
+    s->cur = cur0;
+    s->c0 = c00;
     if (rc) return rc;
     if (ce != hipSuccess) return fail(DGK_ERR_HIP, "graph capture: %s", hipGetErrorString(ce));
-    hipError_t ie = hipGraphInstantiate(&s->graph[c0], g, nullptr, nullptr, 0);
+    hipError_t ie = hipGraphInstantiate(&s->graph[key], g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (ie != hipSuccess) return fail(DGK_ERR_HIP, "graph instantiate: %s", hipGetErrorString(ie));
     s->graph_dt = dt;
   }
-  HIPOK(hipGraphLaunch(s->graph[c0], s->stream));
-  s->cur = c0 ^ 1;
+  HIPOK(hipGraphLaunch(s->graph[key], s->stream));
+  s->cur = s->graph_next[key][0];
+  s->c0 = s->graph_next[key][1];
   return DGK_OK;
 }
 
@@ -1037,11 +1056,11 @@ int dgk_autotune(dgk_solver *s, double dt, int nrep, int *variant_out, double *m
   double *save = nullptr;
   int rc = dalloc(&save, 2 * (size_t)s->Np * (size_t)s->NeA);
   if (rc) return rc;
-  const int cur0 = s->cur;
+  const int cur0 = s->cur, c00 = s->c0;
   int best = -1;
   double best_ms = 0.0;
   if (hipMemcpy(save, s->q[cur0], bytes, hipMemcpyDeviceToDevice) != hipSuccess ||
-      hipMemcpy(save + bytes / sizeof(double), s->q0, bytes, hipMemcpyDeviceToDevice) !=
+      hipMemcpy(save + bytes / sizeof(double), s->q[c00], bytes, hipMemcpyDeviceToDevice) !=
           hipSuccess)
     rc = fail(DGK_ERR_HIP, "autotune: state copy failed");
   for (int v = 0; v < nvar && !rc; ++v) {
@@ -1055,8 +1074,9 @@ int dgk_autotune(dgk_solver *s, double dt, int nrep, int *variant_out, double *m
     }
   }
   s->cur = cur0;
+  s->c0 = c00;
   if (!rc && (hipMemcpy(s->q[cur0], save, bytes, hipMemcpyDeviceToDevice) != hipSuccess ||
-              hipMemcpy(s->q0, save + bytes / sizeof(double), bytes, hipMemcpyDeviceToDevice) !=
+              hipMemcpy(s->q[c00], save + bytes / sizeof(double), bytes, hipMemcpyDeviceToDevice) !=
                   hipSuccess))
     rc = fail(DGK_ERR_HIP, "autotune: state restore failed");
   (void)hipFree(save);

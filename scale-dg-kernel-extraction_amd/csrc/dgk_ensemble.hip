@@ -167,14 +167,17 @@ struct dgk_ensemble {
   std::vector<EnsInst> inst;   // host copy
   std::vector<int> np;         // nodes per element
   long long total = 0;         // doubles per field pool
-  double *q[2] = {nullptr, nullptr};
-  double *q0 = nullptr, *u = nullptr, *v = nullptr, *w = nullptr;
+  // q and q0 rotate among three buffers as in the single solver: stage 1
+  // takes the current buffer as q0, every stage writes the buffer that holds
+  // neither its input nor q0
+  double *q[3] = {nullptr, nullptr, nullptr};
+  double *u = nullptr, *v = nullptr, *w = nullptr;
   int32_t *nbr = nullptr;
   EnsInst *dinst = nullptr;
   EnsOps *dops = nullptr;
   std::vector<EnsOps> hops;    // host copy
-  // stage parameter blocks: [cur parity][stage][instance], rebuilt when dt or
-  // a uniform velocity changes
+  // stage parameter blocks: [3 cur + c0][stage][instance], rebuilt when dt
+  // or a uniform velocity changes
   StageParams *dparams = nullptr;
   int2 *ditems = nullptr;
   int nitems = 0;
@@ -187,12 +190,15 @@ struct dgk_ensemble {
   int slots = 0;                         // resident workgroups (multiple of 8)
   bool tuned = false;
   std::vector<double *> opbufs;
-  int cur = 0;
+  int cur = 0;  // buffer of q
+  int c0 = 2;   // buffer of q0 (never cur)
   hipStream_t stream = nullptr;
   EnsFn fn = nullptr;
   size_t lds = 0;
   int grid = 0;
-  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  // one captured step per starting (cur, c0), and the (cur, c0) it ends in
+  hipGraphExec_t graph[9] = {};
+  int graph_next[9][2] = {};
   bool fields_set = false;
   bool dt_set = false;
 };
@@ -207,37 +213,40 @@ void ens_drop_graphs(dgk_ensemble *s) {
     }
 }
 
-// The stage parameter blocks of every instance for the 2 parities x 3
-// stages: parity c reads q[c] and writes q[c^1] (the ping-pong of the state).
+// The stage parameter blocks of every instance for each placement (cur, c0)
+// of q and q0 and each stage: read q[cur], write the third buffer, q0 is
+// q[c0] (stage 1: q[cur] itself).
 int ens_upload_params(dgk_ensemble *s) {
-  std::vector<StageParams> h((size_t)6 * s->n);
-  for (int c = 0; c < 2; ++c)
-    for (int st = 0; st < 3; ++st)
-      for (int i = 0; i < s->n; ++i) {
-        const EnsInst &I = s->inst[i];
-        const EnsOps &O = s->hops[I.op];
-        StageParams P{};
-        P.qin = s->q[c] + I.off;
-        P.qout = s->q[c ^ 1] + I.off;
-        P.q0 = s->q0 + I.off;
-        P.u = s->u + I.off;
-        P.v = s->v + I.off;
-        P.w = s->w + I.off;
-        P.D = O.D;
-        P.Lift = O.Lift;
-        P.nbr = s->nbr + I.nbr_off;
-        for (int f = 0; f < 6; ++f) P.fs[f] = I.fs[f];
-        for (int d = 0; d < 3; ++d) {
-          P.es[d] = I.es[d];
-          P.cvel[d] = I.cvel[d];
-        }
-        P.Ne = I.ne;
-        P.a = kRkA[st];
-        P.b = kRkB[st];
-        P.dt = I.dt;
-        P.mode = st == 0 ? kModeStage1 : kModeStageN;
-        h[((size_t)c * 3 + st) * s->n + i] = P;
+  std::vector<StageParams> h((size_t)27 * s->n);
+  for (int slot = 0; slot < 27; ++slot) {
+    const int cur = slot / 9, c0 = slot / 3 % 3, st = slot % 3;
+    if (cur == c0) continue;  // no such placement
+    for (int i = 0; i < s->n; ++i) {
+      const EnsInst &I = s->inst[i];
+      const EnsOps &O = s->hops[I.op];
+      StageParams P{};
+      P.qin = s->q[cur] + I.off;
+      P.qout = s->q[3 - cur - c0] + I.off;
+      P.q0 = s->q[st == 0 ? cur : c0] + I.off;
+      P.u = s->u + I.off;
+      P.v = s->v + I.off;
+      P.w = s->w + I.off;
+      P.D = O.D;
+      P.Lift = O.Lift;
+      P.nbr = s->nbr + I.nbr_off;
+      for (int f = 0; f < 6; ++f) P.fs[f] = I.fs[f];
+      for (int d = 0; d < 3; ++d) {
+        P.es[d] = I.es[d];
+        P.cvel[d] = I.cvel[d];
       }
+      P.Ne = I.ne;
+      P.a = kRkA[st];
+      P.b = kRkB[st];
+      P.dt = I.dt;
+      P.mode = st == 0 ? kModeStage1 : kModeStageN;
+      h[(size_t)slot * s->n + i] = P;
+    }
+  }
   HIPOK(hipMemcpy(s->dparams, h.data(), h.size() * sizeof(StageParams), hipMemcpyHostToDevice));
   return DGK_OK;
 }
@@ -247,7 +256,7 @@ EnsArgs ens_args(const dgk_ensemble *s, int stage) {
   A.inst = s->dinst;
   A.ops = s->dops;
   A.items = s->ditems;
-  A.params = s->dparams + ((size_t)s->cur * 3 + stage) * s->n;
+  A.params = s->dparams + ((size_t)(9 * s->cur + 3 * s->c0) + stage) * s->n;
   A.ncls = s->ncls;
   for (int c = 0; c < s->ncls; ++c) A.cls_nq[c] = s->cls_nq[c];
   for (int c = 0; c <= s->ncls; ++c) {
@@ -266,7 +275,11 @@ int ens_launch_args(dgk_ensemble *s, const EnsArgs &A) {
 
 int ens_launch(dgk_ensemble *s, int stage) {
   const int rc = ens_launch_args(s, ens_args(s, stage));
-  if (!rc) s->cur ^= 1;
+  if (!rc) {
+    const int out = 3 - s->cur - s->c0;
+    if (stage == 0) s->c0 = s->cur;
+    s->cur = out;
+  }
   return rc;
 }
 
@@ -324,7 +337,7 @@ extern "C" {
 int dgk_ensemble_destroy(dgk_ensemble *s) {
   if (!s) return DGK_OK;
   ens_drop_graphs(s);
-  void *bufs[] = {s->q[0], s->q[1], s->q0, s->u, s->v, s->w, s->nbr, s->dinst, s->dops, s->ditems,
+  void *bufs[] = {s->q[0], s->q[1], s->q[2], s->u, s->v, s->w, s->nbr, s->dinst, s->dops, s->ditems,
                   s->dparams};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
@@ -461,7 +474,7 @@ int dgk_ensemble_create(dgk_ensemble **out, int n, const int32_t *p, const int32
       if (s->cls_nq[c] == s->inst[i].nq) s->cls_weight[c] += (double)s->inst[i].ne * s->np[i];
   if (!rc) rc = ens_alloc(&s->q[0], off);
   if (!rc) rc = ens_alloc(&s->q[1], off);
-  if (!rc) rc = ens_alloc(&s->q0, off);
+  if (!rc) rc = ens_alloc(&s->q[2], off);
   if (!rc) rc = ens_alloc(&s->u, off);
   if (!rc) rc = ens_alloc(&s->v, off);
   if (!rc) rc = ens_alloc(&s->w, off);
@@ -469,14 +482,14 @@ int dgk_ensemble_create(dgk_ensemble **out, int n, const int32_t *p, const int32
   if (!rc) rc = ens_alloc(&s->dinst, (size_t)n);
   if (!rc) rc = ens_alloc(&s->dops, hops.size());
   if (!rc) rc = ens_alloc(&s->ditems, items.size());
-  if (!rc) rc = ens_alloc(&s->dparams, (size_t)6 * n);
+  if (!rc) rc = ens_alloc(&s->dparams, (size_t)27 * n);
   if (!rc && (hipMemcpy(s->nbr, nbr.data(), nbr.size() * sizeof(int32_t), hipMemcpyHostToDevice) ||
               hipMemcpy(s->dops, hops.data(), hops.size() * sizeof(EnsOps), hipMemcpyHostToDevice) ||
               hipMemcpy(s->ditems, items.data(), items.size() * sizeof(int2),
                         hipMemcpyHostToDevice) ||
               hipMemcpy(s->dinst, s->inst.data(), n * sizeof(EnsInst), hipMemcpyHostToDevice)))
     rc = fail(DGK_ERR_HIP, "ensemble table upload failed");
-  if (!rc && hipMemset(s->q0, 0, off * sizeof(double)) != hipSuccess)
+  if (!rc && hipMemset(s->q[2], 0, off * sizeof(double)) != hipSuccess)
     rc = fail(DGK_ERR_HIP, "hipMemset failed");
   if (!rc) rc = ens_upload_params(s);
   if (!rc) {
@@ -552,22 +565,26 @@ int dgk_ensemble_step(dgk_ensemble *s) {
   if (!s) return fail(DGK_ERR_ARG, "null ensemble");
   if (!s->fields_set) return fail(DGK_ERR_STATE, "no fields set");
   if (!s->dt_set) return fail(DGK_ERR_STATE, "no time steps set (dgk_ensemble_set_dt)");
-  const int c0 = s->cur;
-  if (!s->graph[c0]) {
+  const int key = 3 * s->cur + s->c0, cur0 = s->cur, c00 = s->c0;
+  if (!s->graph[key]) {
     hipGraph_t g;
     HIPOK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
     int rc = DGK_OK;
     for (int st = 0; st < 3 && !rc; ++st) rc = ens_launch(s, st);
     hipError_t ce = hipStreamEndCapture(s->stream, &g);
-    s->cur = c0;
+    s->graph_next[key][0] = s->cur;
+    s->graph_next[key][1] = s->c0;
+    s->cur = cur0;
+    s->c0 = c00;
     if (rc) return rc;
     if (ce != hipSuccess) return fail(DGK_ERR_HIP, "graph capture: %s", hipGetErrorString(ce));
-    hipError_t ie = hipGraphInstantiate(&s->graph[c0], g, nullptr, nullptr, 0);
+    hipError_t ie = hipGraphInstantiate(&s->graph[key], g, nullptr, nullptr, 0);
     (void)hipGraphDestroy(g);
     if (ie != hipSuccess) return fail(DGK_ERR_HIP, "graph instantiate: %s", hipGetErrorString(ie));
   }
-  HIPOK(hipGraphLaunch(s->graph[c0], s->stream));
-  s->cur = c0 ^ 1;
+  HIPOK(hipGraphLaunch(s->graph[key], s->stream));
+  s->cur = s->graph_next[key][0];
+  s->c0 = s->graph_next[key][1];
   return DGK_OK;
 }
 
@@ -611,9 +628,8 @@ int dgk_ensemble_tune(dgk_ensemble *s, int nrep, double *ms_per_class) {
   int rc = DGK_OK;
   const int units = std::max(s->slots / kNumXcd, 1);
   for (int c = 0; c < s->ncls && !rc; ++c) {
-    // stage 1 of class c alone on the whole grid: reads q, writes q0 (= q,
-    // as the next step's first stage does) and the unused ping-pong buffer,
-    // so the state is left as it was
+    // stage 1 of class c alone on the whole grid: reads q and writes the
+    // buffer that holds neither q nor q0, so the state is left as it was
     EnsArgs A = ens_args(s, 0);
     const int u = std::min(units, ens_unit_cap(s, c));
     A.ncls = 1;

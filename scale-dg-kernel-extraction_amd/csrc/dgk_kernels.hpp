@@ -28,13 +28,13 @@ constexpr int kNumXcd = 8;   // MI355X: 8 XCDs, workgroups dispatched round-robi
 
 // Stage modes of stage_kernel.
 constexpr int kModeTend = 0;    // write dqdt only
-constexpr int kModeStage1 = 1;  // q0 <- q, q <- rk_a q0 + rk_b (q + dt dqdt)
+constexpr int kModeStage1 = 1;  // q <- rk_a q + rk_b (q + dt dqdt); the input becomes q0
 constexpr int kModeStageN = 2;  // q <- rk_a q0 + rk_b (q + dt dqdt), q0 read
 
 struct StageParams {
   const double *qin;   // (Np,NeA) current state, halo valid in table mode
   double *qout;        // (Np,NeA) next state (owned part written)
-  double *q0;          // (Np,NeA) RK stage-0 copy
+  const double *q0;    // (Np,NeA) state at the start of the step (stages 2, 3)
   const double *u, *v, *w;
   double *dqdt;        // (Np,NeA) tendency (kModeTend)
   const double *D;     // (Nq,Nq) D1D column-major
@@ -81,6 +81,12 @@ __device__ __forceinline__ long long tiled_element(const StageParams &P, int x, 
   const long long i = ti * P.ttx + il, j = tj * P.tty + jl;
   if (i >= P.tnx || j >= P.tny) return -1;
   return i + (long long)P.tnx * (j + (long long)P.tny * k);
+}
+
+// q0 is read once per stage and by no neighbour: a non-temporal load keeps
+// it from displacing lines that neighbouring elements read again from L2.
+__device__ __forceinline__ double ld_stream(const double *p) {
+  return __builtin_nontemporal_load(p);
 }
 
 // ---------------------------------------------------------------------------
@@ -252,7 +258,7 @@ __global__ void __launch_bounds__(StageShape<NQ>::NT)
         xu[r] = P.u[idx];
         xv[r] = P.v[idx];
         xw[r] = P.w[idx];
-        xq0[r] = need_q0 ? P.q0[idx] : 0.0;
+        xq0[r] = need_q0 ? ld_stream(P.q0 + idx) : 0.0;
       }
     }
   };
@@ -335,7 +341,6 @@ __global__ void __launch_bounds__(StageShape<NQ>::NT)
           P.dqdt[idx] = dq;
         } else {
           const double q0s = P.mode == kModeStage1 ? q : q0v;
-          if (P.mode == kModeStage1) P.q0[idx] = q;
           double qn;
           if constexpr (FMA) qn = __builtin_fma(P.a, q0s, P.b * __builtin_fma(P.dt, dq, q));
           else qn = P.a * q0s + P.b * (q + P.dt * dq);
@@ -352,7 +357,7 @@ __global__ void __launch_bounds__(StageShape<NQ>::NT)
         const int m = tid + r * NT;
         const long long idx = g * NN + m;
         const bool ok = m < NN && g * EPB + m / NP < P.Ne;
-        node(r, ok ? P.qin[idx] : 0.0, ok && need_q0 ? P.q0[idx] : 0.0);
+        node(r, ok ? P.qin[idx] : 0.0, ok && need_q0 ? ld_stream(P.q0 + idx) : 0.0);
       }
     }
     if constexpr (HOLD) {

@@ -434,7 +434,7 @@ __device__ __forceinline__ void pencil_load(PT &P, long long eb, int pl, PencilL
       }
       L.w[k] = P.w[base + NFP * k];
     }
-    if constexpr (Q0) L.q0[k] = need_q0 ? P.q0[base + NFP * k] : 0.0;
+    if constexpr (Q0) L.q0[k] = need_q0 ? ld_stream(P.q0 + base + NFP * k) : 0.0;
   }
   const int32_t *nb = P.nbr + eb * 6;
   // z faces at (i,j): -z (face 4) sees the neighbour's k = NQ-1, +z (face 5)
@@ -491,7 +491,7 @@ __device__ __forceinline__ void pencil_compute(PT &P, const double *__restrict__
   double (&q0)[NQ] = L.q0;
   if constexpr (LATE_Q0 && !LEAN) {
 #pragma unroll
-    for (int k = 0; k < NQ; ++k) q0[k] = need_q0 ? P.q0[base + NFP * k] : 0.0;
+    for (int k = 0; k < NQ; ++k) q0[k] = need_q0 ? ld_stream(P.q0 + base + NFP * k) : 0.0;
   }
   const double qP4 = L.qP4, qP5 = L.qP5, qP0 = L.qP0, qP2 = L.qP2, qP3 = L.qP3, qP1 = L.qP1;
   const double wP4 = L.wP4, wP5 = L.wP5, vP0 = L.vP0, vP2 = L.vP2, uP3 = L.uP3, uP1 = L.uP1;
@@ -622,7 +622,7 @@ __device__ __forceinline__ void pencil_compute(PT &P, const double *__restrict__
   const int i = la, j = lb;
   if constexpr (LEAN) {
 #pragma unroll
-    for (int k = 0; k < NQ; ++k) q0[k] = need_q0 ? P.q0[base + NFP * k] : 0.0;
+    for (int k = 0; k < NQ; ++k) q0[k] = need_q0 ? ld_stream(P.q0 + base + NFP * k) : 0.0;
 #pragma unroll
     for (int k = 0; k < NQ; ++k) q[k] = Bq[pidx<NQ>(pl + NFP * k)];
     z_phase(q);
@@ -701,7 +701,6 @@ __device__ __forceinline__ void pencil_compute(PT &P, const double *__restrict__
     } else {
       const double qq = q[k];
       const double qs = P.mode == kModeStage1 ? qq : q0[k];
-      if (P.mode == kModeStage1) st<NT>(P.q0 + idx, qq);
       double qn;
       if constexpr (FMA) qn = __builtin_fma(P.a, qs, P.b * __builtin_fma(P.dt, dq, qq));
       else qn = P.a * qs + P.b * (qq + P.dt * dq);
