@@ -385,13 +385,15 @@ bool tiled_order(const dgk_solver *s) {
          !((s->variant >= 0 ? s->variant : 0) & kPencilLinearOrder);
 }
 
-// Tiles 32 elements wide: x-neighbours' face traces (a 64-byte-strided
-// gather that touches every line of the neighbour) are then read from L2 for
-// all but 1/32 of the elements; at 64^3, p=7 wider or narrower tiles cost
-// 2-7 % more.
+// Tiles of up to 32x32 elements: x-neighbours' face traces (a 64-byte-
+// strided gather that touches every line of the neighbour) are then read from
+// L2 for all but 1/32 of the elements; at 64^3, p=7 tiles 16x8, 32x8, 32x16
+// and 64x16 measured 1-8 % slower.  The extent is split into equal tiles
+// (48 -> 2 x 24), so that few tile slots fall outside the mesh.
 void tile_shape(const dgk_solver *s, int *tx, int *ty, long long *tseq) {
-  *tx = std::min(32, s->NeX);
-  *ty = std::min(16, s->NeY);
+  auto even = [](int n) { return (n + (n + 31) / 32 - 1) / ((n + 31) / 32); };
+  *tx = even(s->NeX);
+  *ty = even(s->NeY);
   const long long ntx = (s->NeX + *tx - 1) / *tx, nty = (s->NeY + *ty - 1) / *ty;
   const long long nk = (s->NeZ + kNumXcd - 1) / kNumXcd;
   *tseq = ntx * nty * (long long)(*tx) * (*ty) * nk;

@@ -59,7 +59,7 @@ struct EnsArgs {
   int cls_nq[kEnsMaxClass];
   int cls_block[kEnsMaxClass + 1];  // first workgroup of each class (multiples of 8)
   int cls_item[kEnsMaxClass + 1];   // first item of each class
-  const StageParams *params;        // per instance, for this launch's stage and parity
+  const StageParams *params;        // per instance, for this launch's stage and placement
 };
 
 bool ens_supported(int nq) { return nq == 2 || nq == 4 || nq == 6 || nq == 8 || nq == 12; }

@@ -375,6 +375,39 @@ def test_graph_step_equals_stagewise(gpu):
     b.close()
 
 
+def test_q0_placement_and_stage_sequence(gpu):
+    """Stage 1 keeps the buffer it read as q0 instead of copying it: q0 then
+    holds the state the step started from through the later stages, q and
+    q0 never share a buffer, and any sequence of stages combines with that
+    q0 -- each stage equals q_new = a q0 + b (q + dt dqdt) evaluated on the
+    host from the device tendency (mod_common.f90:29-30, main.f90:53-73)."""
+    ne, p = (3, 4, 2), 3
+    rk_a, rk_b = (0.0, 0.75, 1.0 / 3.0), (1.0, 0.25, 2.0 / 3.0)
+    x1D, D, L = dgk.read_operator(p)
+    m = dgk.mesh_setup(*ne, p, x1D=x1D, tables=False)
+    q = ic.gaussian_noise(m["pos_en"], 47)
+    vel = [np.full_like(q, c) for c in (0.3, -0.8, 0.5)]
+    dt = ic.cfl_dt(p, (0.3, -0.8, 0.5), ne)
+    s = dgk.Solver.structured(*ne, p, D, L)
+    t = dgk.Solver.structured(*ne, p, D, L)
+    s.set_fields(q, *vel)
+    cur, q0 = s.get_field(), None
+    for n, st in enumerate((1, 2, 2, 3, 1, 3, 3)):
+        say(f"[q0] stage {st} ({n})")
+        t.set_fields(cur, *vel)
+        dq = t.cal_tend()
+        if st == 1:
+            q0 = cur
+        want = rk_a[st - 1] * q0 + rk_b[st - 1] * (cur + dt * dq)
+        s.rk_stage(st, dt)
+        cur = s.get_field()
+        assert np.array_equal(cur, want), (n, st)
+        assert np.array_equal(s.get_field(dgk.FIELD_Q0), q0), (n, st)
+        assert s.field_ptr(dgk.FIELD_Q) != s.field_ptr(dgk.FIELD_Q0)
+    s.close()
+    t.close()
+
+
 def test_state_errors(gpu):
     _, D, L = dgk.read_operator(2)
     s = dgk.Solver.structured(2, 2, 2, 2, D, L)
