@@ -390,7 +390,8 @@ def main():
         "exact_variant": None if ms_exact is None else {
             "kernel_us": round(ms_exact * 1e3, 2),
             "gdof_updates_per_s": round(dof / (ms_exact * 1e-3) / 1e9, 2),
-            "hbm_frac": round(bytes_launch / (ms_exact * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+            # the bit-identical variant streams per-node velocity
+            "hbm_frac": round(BYTES_PER_DOF * dof / (ms_exact * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
     }
     print(json.dumps(out), flush=True)
 
