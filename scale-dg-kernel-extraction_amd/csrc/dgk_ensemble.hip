@@ -68,9 +68,10 @@ bool ens_supported(int nq) { return nq == 2 || nq == 4 || nq == 6 || nq == 8 || 
 // and 1 elements for Nq 2, 4, 6, 8, 12.
 int ens_item_elems(int nq) { return kEnsThreads / (nq * nq); }
 
+// (the ensemble stages q in LDS whatever the velocity)
 template <int NQ>
-size_t ens_lds_nq(bool cv) {
-  return sizeof(double) * (kEnsThreads / (NQ * NQ)) * PencilShape<NQ>::LEAN_ELEM;
+size_t ens_lds_nq(bool) {
+  return sizeof(double) * (kEnsThreads / (NQ * NQ)) * PencilShape<NQ>::lean_elem(false);
 }
 
 size_t ens_item_lds(int nq, bool cv) {
@@ -100,7 +101,7 @@ template <int NQ, int ORDER, bool FMA, bool FACT, bool CV>
 __device__ __forceinline__ void ens_items(const EnsArgs &A, int first, int count, int lb, int nb,
                                           double *smem) {
   constexpr int NFP = NQ * NQ, EPB = kEnsThreads / NFP;
-  constexpr int PER = PencilShape<NQ>::LEAN_ELEM;
+  constexpr int PER = PencilShape<NQ>::lean_elem(false);  // q staged in LDS
   const int xcd = lb % kNumXcd, slot = lb / kNumXcd, nslot = nb / kNumXcd;
   const int per = (count + kNumXcd - 1) / kNumXcd;
   const int end = first + min(count, (xcd + 1) * per);

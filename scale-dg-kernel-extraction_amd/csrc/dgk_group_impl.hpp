@@ -30,7 +30,7 @@ StageKernel make_group(int epb) {
   k.launch = launch_group<NQ, ORDER, FMA, FACT, CV>;
   k.epb = epb;
   k.nt = pencil_group_threads<NQ>(epb);
-  k.lds = pencil_group_lds<NQ>(epb);
+  k.lds = pencil_group_lds<NQ>(epb, CV);
   k.pencil = true;
   return k;
 }

@@ -40,7 +40,14 @@ struct PencilShape {
   static constexpr size_t LDS = sizeof(double) * EPW * PER_ELEM;
   // LDS of one element in the lean layout: two volume buffers (q and the
   // partial sum S) and the four x/y face-flux buffers.
-  static constexpr int LEAN_ELEM = 2 * PAD + 4 * FPAD;
+  // With uniform velocity and Nq >= 10 the group kernel's lean stage reads
+  // the x- and y-lines of q from memory as well (lean_qg): the partial sum S
+  // is then the only volume buffer, the element needs 40 % less LDS and
+  // the wavefronts hold q in no register.  Measured 3-11 % faster at
+  // Nq = 10, 12, 13, 16; 5 % slower at Nq = 14, with per-node velocity and
+  // at lower orders, and in the ensemble (whose occupancy is set by VGPRs).
+  static constexpr bool lean_qg(bool cv) { return cv && NQ >= 10 && NQ != 14; }
+  static constexpr int lean_elem(bool cv) { return (lean_qg(cv) ? 1 : 2) * PAD + 4 * FPAD; }
 };
 
 // Threads and LDS of a multi-wavefront pencil workgroup of epb elements:
@@ -49,8 +56,8 @@ struct PencilShape {
 template <int NQ>
 constexpr int pencil_group_threads(int epb) { return (epb * NQ * NQ + 63) / 64 * 64; }
 template <int NQ>
-constexpr size_t pencil_group_lds(int epb) {
-  return sizeof(double) * epb * PencilShape<NQ>::LEAN_ELEM;
+constexpr size_t pencil_group_lds(int epb, bool cv) {
+  return sizeof(double) * epb * PencilShape<NQ>::lean_elem(cv);
 }
 
 template <int NQ>
@@ -412,9 +419,10 @@ struct PencilLoads {
 };
 
 // Q0: also load q0 (stages 2 and 3 need it).  UV: also load the z-lines of
-// u and v (the lean layout reads their x- and y-lines itself).  eb is a
-// valid element.
-template <int NQ, bool CV, bool Q0, bool UV, class PT>
+// u and v (the lean layout reads their x- and y-lines itself).  QZ: load the
+// z-line of q (the lean layout with q lines from memory reads it at the
+// end).  eb is a valid element.
+template <int NQ, bool CV, bool Q0, bool UV, class PT, bool QZ = true>
 __device__ __forceinline__ void pencil_load(PT &P, long long eb, int pl, PencilLoads<NQ> &L) {
   constexpr int NP = NQ * NQ * NQ, NFP = NQ * NQ;
   const int la = pl % NQ, lb = pl / NQ;
@@ -422,7 +430,7 @@ __device__ __forceinline__ void pencil_load(PT &P, long long eb, int pl, PencilL
   const bool need_q0 = P.mode == kModeStageN;
 #pragma unroll
   for (int k = 0; k < NQ; ++k) {
-    L.q[k] = P.qin[base + NFP * k];
+    if constexpr (QZ) L.q[k] = P.qin[base + NFP * k];
     if constexpr (CV) {
       L.u[k] = P.cvel[0];
       L.v[k] = P.cvel[1];
@@ -464,7 +472,7 @@ __device__ __forceinline__ void pencil_load(PT &P, long long eb, int pl, PencilL
 // The stage of one element from its loaded lines.  LATE_Q0: q0 is not in L
 // and is read here, at the start (non-LEAN) or in the final phase (LEAN).
 template <int NQ, int ORDER, bool FMA, bool FACT_LIFT, bool NT, bool CV, bool LEAN, bool LATE_Q0,
-          class PT>
+          class PT, bool QG = false>
 __device__ __forceinline__ void pencil_compute(PT &P, const double *__restrict__ Drow,
                                                const double *__restrict__ Lift,
                                                const double *__restrict__ lfac, double *blk,
@@ -473,12 +481,14 @@ __device__ __forceinline__ void pencil_compute(PT &P, const double *__restrict__
   // Lift: the pair-packed copy described at the lift loads below
   using S = PencilShape<NQ>;
   constexpr int NP = S::NP, NFP = S::NFP, PAD = S::PAD, FPAD = S::FPAD;
-  constexpr int NVOL = LEAN ? 2 : 3;
+  // QG (lean only): the x- and y-lines of q are read from memory, not LDS
+  static_assert(!QG || LEAN, "q lines from memory need the lean layout");
+  constexpr int NVOL = LEAN ? (QG ? 1 : 2) : 3;
   const int la = pl % NQ, lb = pl / NQ;
   double *Bq = blk;
   double *Bu = Bq + PAD;  // u, then D_x (LEAN: the partial sum S)
   double *Bv = Bu + PAD;  // v, then D_y
-  double *BS = Bu;        // LEAN: Es1 D_x, then + Es2 D_y
+  double *BS = QG ? blk : Bu;  // LEAN: Es1 D_x, then + Es2 D_y
   double *Bf = Bq + NVOL * PAD;  // fluxes of faces -y, +y, -x, +x
   const bool valid = e >= 0;
   const long long ebase = eb * NP;
@@ -510,18 +520,20 @@ __device__ __forceinline__ void pencil_compute(PT &P, const double *__restrict__
     flux5 = upwind_flux(qz[NQ - 1], qP5, w[NQ - 1], wP5, P.fs[5]);
   };
   if constexpr (!LEAN) z_phase(q);
-  if (act) {
+  if constexpr (!QG) {
+    if (act) {
 #pragma unroll
-    for (int k = 0; k < NQ; ++k) {
-      const int s = pidx<NQ>(pl + NFP * k);
-      Bq[s] = q[k];
-      if constexpr (!CV && !LEAN) {
-        Bu[s] = u[k];
-        Bv[s] = v[k];
+      for (int k = 0; k < NQ; ++k) {
+        const int s = pidx<NQ>(pl + NFP * k);
+        Bq[s] = q[k];
+        if constexpr (!CV && !LEAN) {
+          Bu[s] = u[k];
+          Bv[s] = v[k];
+        }
       }
     }
+    __syncthreads();
   }
-  __syncthreads();
   const double e1s = P.es[0], e2s = P.es[1], e3s = P.es[2];
 
   if constexpr (LEAN) {
@@ -533,7 +545,7 @@ __device__ __forceinline__ void pencil_compute(PT &P, const double *__restrict__
       for (int l = 0; l < NQ; ++l) ux[l] = CV ? P.cvel[0] : P.u[ebase + l + NQ * pl];
 #pragma unroll
       for (int l = 0; l < NQ; ++l) {
-        qx[l] = Bq[pidx<NQ>(l + NQ * pl)];
+        qx[l] = QG ? P.qin[ebase + l + NQ * pl] : Bq[pidx<NQ>(l + NQ * pl)];
         fx[l] = qx[l] * ux[l];
       }
       sweep<NQ, ORDER, FMA, true>(Drow, fx, dx);
@@ -555,7 +567,7 @@ __device__ __forceinline__ void pencil_compute(PT &P, const double *__restrict__
       for (int l = 0; l < NQ; ++l) vy[l] = CV ? P.cvel[1] : P.v[ebase + la + NQ * l + NFP * lb];
 #pragma unroll
       for (int l = 0; l < NQ; ++l) {
-        qy[l] = Bq[pidx<NQ>(la + NQ * l + NFP * lb)];
+        qy[l] = QG ? P.qin[ebase + la + NQ * l + NFP * lb] : Bq[pidx<NQ>(la + NQ * l + NFP * lb)];
         fy[l] = qy[l] * vy[l];
       }
       sweep<NQ, ORDER, FMA, false>(Drow, fy, dy);
@@ -624,7 +636,7 @@ __device__ __forceinline__ void pencil_compute(PT &P, const double *__restrict__
 #pragma unroll
     for (int k = 0; k < NQ; ++k) q0[k] = need_q0 ? ld_stream(P.q0 + base + NFP * k) : 0.0;
 #pragma unroll
-    for (int k = 0; k < NQ; ++k) q[k] = Bq[pidx<NQ>(pl + NFP * k)];
+    for (int k = 0; k < NQ; ++k) q[k] = QG ? P.qin[base + NFP * k] : Bq[pidx<NQ>(pl + NFP * k)];
     z_phase(q);
   }
 #pragma unroll
@@ -723,7 +735,7 @@ __device__ __forceinline__ void pencil_compute(PT &P, const double *__restrict__
 // PT: the parameter block's type, StageParams or a StageParams in the
 // constant address space (read on demand with scalar loads).
 template <int NQ, int ORDER, bool FMA, bool FACT_LIFT, bool NT, bool CV, bool LEAN = false,
-          class PT = const StageParams>
+          class PT = const StageParams, bool QG = false>
 __device__ __forceinline__ void pencil_element(PT &P, const double *__restrict__ Drow,
                                                const double *__restrict__ Lift,
                                                const double *__restrict__ lfac, double *blk,
@@ -732,9 +744,9 @@ __device__ __forceinline__ void pencil_element(PT &P, const double *__restrict__
   const long long eb = e >= 0 ? e : e_any;  // a valid element for address math
   PencilLoads<NQ> L;
   // LEAN reads q0 in the final phase
-  pencil_load<NQ, CV, !LEAN, !LEAN>(P, eb, pl, L);
-  pencil_compute<NQ, ORDER, FMA, FACT_LIFT, NT, CV, LEAN, false>(P, Drow, Lift, lfac, blk, pl, e,
-                                                                 eb, act, L);
+  pencil_load<NQ, CV, !LEAN, !LEAN, PT, !QG>(P, eb, pl, L);
+  pencil_compute<NQ, ORDER, FMA, FACT_LIFT, NT, CV, LEAN, false, PT, QG>(P, Drow, Lift, lfac, blk,
+                                                                         pl, e, eb, act, L);
 }
 
 template <int NQ, int ORDER, bool FMA, bool FACT_LIFT, bool NT, bool CV>
@@ -784,7 +796,7 @@ __global__ void __launch_bounds__(MAXTH)
                         const double *__restrict__ Lift, const double *__restrict__ lfac,
                         int epb) {
   constexpr int NFP = NQ * NQ;
-  constexpr int PER = PencilShape<NQ>::LEAN_ELEM;
+  constexpr int PER = PencilShape<NQ>::lean_elem(CV);
   extern __shared__ double smem[];
   const int tid = threadIdx.x;
   const bool act = tid < epb * NFP;
@@ -808,8 +820,9 @@ __global__ void __launch_bounds__(MAXTH)
     if (!act || e >= P.Ne) e = -1;
     e_any = gl * epb;
   }
-  pencil_element<NQ, ORDER, FMA, FACT_LIFT, NT, CV, true>(P, Drow, Lift, lfac, smem + el * PER,
-                                                          pl, e, e_any, act);
+  pencil_element<NQ, ORDER, FMA, FACT_LIFT, NT, CV, true, const StageParams,
+                 PencilShape<NQ>::lean_qg(CV)>(P, Drow, Lift, lfac, smem + el * PER, pl, e,
+                                               e_any, act);
 }
 
 }  // namespace dgk
