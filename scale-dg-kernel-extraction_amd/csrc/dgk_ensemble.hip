@@ -125,8 +125,11 @@ __device__ __forceinline__ void ens_items(const EnsArgs &A, int first, int count
   }
 }
 
+// At least three workgroups (12 wavefronts) per CU: every form then fits in
+// 168 VGPRs without spilling (the per-node-velocity FMA form would take 174
+// and two workgroups).
 template <int ORDER, bool FMA, bool FACT, bool CV>
-__global__ void __launch_bounds__(kEnsThreads) ensemble_kernel(const EnsArgs A) {
+__global__ void __launch_bounds__(kEnsThreads, 3) ensemble_kernel(const EnsArgs A) {
   extern __shared__ double smem[];
   int c = 0;
   while (c + 1 < A.ncls && (int)blockIdx.x >= A.cls_block[c + 1]) ++c;
