@@ -350,6 +350,10 @@ def main():
     extra = {}
     if not args.no_extra:
         extra["c3_64cube_p7_mixed"] = mixed_workload(args.workload, x1D, D, L, spec)
+        # the same mesh with the uniform velocity held as three scalars, and
+        # the FMA form with per-node velocity (each autotuned)
+        extra["c3_64cube_p7_other_variants"] = single_workload(
+            "c3_64cube_p7", ("factored_constvel", "fma_factored_constvel", "fma_factored"))
         extra["c5_16cube_p15"] = single_workload("c5_16cube_p15", ("exact", "factored",
                                                                    "factored_constvel",
                                                                    "fma_factored_constvel"))
