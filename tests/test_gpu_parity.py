@@ -294,6 +294,30 @@ def test_stretched_mesh_keeps_tables(gpu):
     s.close()
 
 
+@pytest.mark.parametrize("p,ne", [(1, (33, 34, 9)), (2, (40, 3, 8)), (3, (65, 2, 8)),
+                                  (7, (35, 2, 8)), (11, (2, 33, 8))])
+def test_tiled_order_with_several_tiles(gpu, p, ne):
+    """Extents above 32 elements are split into several (uneven) tiles per
+    axis, with tile slots outside the mesh (33 -> 2 x 17, 65 -> 3 x 22):
+    every element is still updated once, with the reference's bits."""
+    nstep, vel, kind = 2, (0.4, -0.3, 0.6), "rand"
+    x1D, D, L = dgk.read_operator(p)
+    m = dgk.mesh_setup(*ne, p, x1D=x1D, tables=False)
+    q, u, v, w, dt = CS.run_inputs(ne, p, vel, kind, m["pos_en"])
+    om = O.Mesh(*ne, p)
+    F = oracle_fields(om, q, u, v, w)
+    om.run(*F, dt, nstep, O.OPT1)
+    s = dgk.Solver.structured(*ne, p, D, L)
+    assert s.query(9) == 1  # tiled order
+    for var in (0, 1):
+        say(f"[tiles] {ne} p={p} variant={var}")
+        s.set_variant(var)
+        s.set_fields(q, u, v, w)
+        s.run(dt, nstep)
+        assert np.array_equal(s.get_field(), F[0][:, :om.Ne]), var
+    s.close()
+
+
 @pytest.mark.parametrize("p", range(1, 16))
 @pytest.mark.parametrize("ne", [(3, 2, 9), (4, 5, 3)])
 def test_every_tuning_variant_is_bit_exact(gpu, p, ne):
