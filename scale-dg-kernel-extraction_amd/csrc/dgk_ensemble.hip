@@ -7,9 +7,9 @@
 // into classes of equal order; the persistent grid is split between the
 // classes (in units of eight workgroups, one per XCD) and every workgroup
 // runs items of one class only, so that it is compiled and scheduled as one
-// straight code path.  A workgroup of 256 threads runs an item as the pencil
-// stage (one lane per node line) of 64, 16, 7, 4 or 1 elements for Nq = 2,
-// 4, 6, 8, 12.
+// straight code path.  A workgroup of 192 threads (three wavefronts, each on
+// its own SIMD) runs an item as the pencil stage (one lane per node line) of
+// 24, 12, 5, 3 or 1 elements for Nq = 2, 4, 6, 8, 12.
 // Within a class the items are split into eight contiguous ranges, one per
 // XCD, so that an instance's neighbouring elements stay in one L2.  The
 // split of the grid between the classes follows measured per-class stage
@@ -34,7 +34,7 @@ using namespace dgk;
 
 namespace {
 
-constexpr int kEnsThreads = 256;
+constexpr int kEnsThreads = 192;
 
 struct EnsInst {
   int nq, op;            // Nq and operator slot
@@ -64,7 +64,7 @@ struct EnsArgs {
 
 bool ens_supported(int nq) { return nq == 2 || nq == 4 || nq == 6 || nq == 8 || nq == 12; }
 
-// Elements per item (one workgroup): Nq^2 lanes per element, 64, 16, 7, 4
+// Elements per item (one workgroup): Nq^2 lanes per element, 24, 12, 5, 3
 // and 1 elements for Nq 2, 4, 6, 8, 12.
 int ens_item_elems(int nq) { return kEnsThreads / (nq * nq); }
 
@@ -125,9 +125,10 @@ __device__ __forceinline__ void ens_items(const EnsArgs &A, int first, int count
   }
 }
 
-// At least three workgroups (12 wavefronts) per CU: every form then fits in
-// 168 VGPRs without spilling (the per-node-velocity FMA form would take 174
-// and two workgroups).
+// At least three wavefronts per SIMD (the second launch bound): every form
+// then fits in 168 VGPRs without spilling (the per-node-velocity FMA form
+// would take 174 and hold two).  Four workgroups of three wavefronts fill
+// the CU's twelve.
 template <int ORDER, bool FMA, bool FACT, bool CV>
 __global__ void __launch_bounds__(kEnsThreads, 3) ensemble_kernel(const EnsArgs A) {
   extern __shared__ double smem[];
