@@ -377,12 +377,28 @@ struct dgk_solver {
 
 namespace {
 
+// Built-in default tuning variant (overridden by dgk_set_variant):
+// non-temporal streaming stores measured fastest at Nq = 8 (64^3 elements),
+// and with uniform velocity there also the linear element order (the
+// autotuner's pick, 5-10 % faster than the tiled one); at Nq = 2 (64-byte
+// elements, 16 per wavefront) plain stores, 17 % faster.
+int default_pencil_variant(const dgk_solver *s) {
+  if (s->Nq == 2) return 0;
+  if (s->Nq == 8 && (s->flags & DGK_FLAG_CONST_VELOCITY) && s->order != kMixed)
+    return kPencilNtStore | kPencilLinearOrder;
+  return kPencilNtStore;
+}
+
+int current_variant(const dgk_solver *s) {
+  return s->variant >= 0 ? s->variant : default_pencil_variant(s);
+}
+
 // Tiled element order of the pencil kernel (tiled_element): on for a known
 // structured numbering with at least one z-plane per XCD, unless the tuning
 // variant asks for the linear order.
 bool tiled_order(const dgk_solver *s) {
   return s->sk.pencil && s->NeX > 0 && s->NeZ >= kNumXcd &&
-         !((s->variant >= 0 ? s->variant : 0) & kPencilLinearOrder);
+         !(current_variant(s) & kPencilLinearOrder);
 }
 
 // Tiles of up to 32x32 elements: x-neighbours' face traces (a 64-byte-
@@ -399,13 +415,6 @@ void tile_shape(const dgk_solver *s, int *tx, int *ty, long long *tseq) {
   *tseq = ntx * nty * (long long)(*tx) * (*ty) * nk;
 }
 
-// Built-in default tuning variant per Nq (overridden by dgk_set_variant):
-// non-temporal streaming stores measured fastest at Nq = 8 (64^3 elements);
-// at Nq = 2 (64-byte elements, 16 per wavefront) plain stores, 17 % faster.
-int default_pencil_variant(int Nq) {
-  return Nq == 2 ? 0 : kPencilNtStore;
-}
-
 // Tuning variants of the solver's stage kernel: store flavour x element
 // order x workgroup shape (Nq 13-16 have one shape).
 int pencil_variants(const dgk_solver *s) {
@@ -416,7 +425,7 @@ int pencil_variants(const dgk_solver *s) {
 int configure_launch(dgk_solver *s) {
   const bool use_pencil =
       s->structured && !(s->flags & DGK_FLAG_NODE_KERNEL) && pencil_nq(s->Nq);
-  const int var = s->variant >= 0 ? s->variant : default_pencil_variant(s->Nq);
+  const int var = current_variant(s);
   if (s->order == kMixed && !use_pencil)
     return fail(DGK_ERR_ARG, "DGK_FLAG_MIXED needs the one-wavefront pencil kernel (Nq 2, 4 "
                              "or 8, compressed connectivity) without DGK_FLAG_FMA");
@@ -987,7 +996,7 @@ int dgk_query(dgk_solver *s, int what, int64_t *value) {
   case 5: *value = s->sk.nt; break;
   case 6: *value = s->sk.pencil ? 1 : 0; break;
   case 7: *value = (int64_t)s->sk.lds; break;
-  case 8: *value = s->variant >= 0 ? s->variant : default_pencil_variant(s->Nq); break;
+  case 8: *value = current_variant(s); break;
   case 9: *value = tiled_order(s) ? 1 : 0; break;
   default: return fail(DGK_ERR_ARG, "unknown query %d", what);
   }
