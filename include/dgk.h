@@ -259,8 +259,8 @@ int dgk_divlike_dirXYZ_var5_batched(int Nq, int nelem, const double *Mat,
  * initial state and dt, advanced together: one persistent launch per RK
  * stage over all instances, the three stages of a step captured in one
  * hipGraph.  Every instance gives the same bits as a dgk_create_structured
- * solver with the same flags (OPT1 summation order).  PolyOrder 1, 3, 5, 7
- * or 11.  dom: n triples of domain lengths, or NULL for unit cubes.
+ * solver with the same flags (OPT1 summation order).  PolyOrder 1..12.
+ * dom: n triples of domain lengths, or NULL for unit cubes.
  * operator_dir: the directory of the reference's pN.dat files.  flags:
  * DGK_FLAG_FMA, DGK_FLAG_FACTORED_LIFT, DGK_FLAG_CONST_VELOCITY. */
 typedef struct dgk_ensemble dgk_ensemble;

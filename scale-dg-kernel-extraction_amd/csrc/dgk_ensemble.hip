@@ -49,7 +49,7 @@ struct EnsOps {          // per distinct Nq
   const double *D, *Lift, *Drow, *LiftP, *lfac;
 };
 
-constexpr int kEnsMaxClass = 5;  // Nq 2, 4, 6, 8, 12
+constexpr int kEnsMaxClass = 12;  // Nq 2..13
 
 struct EnsArgs {
   const EnsInst *inst;
@@ -62,7 +62,8 @@ struct EnsArgs {
   const StageParams *params;        // per instance, for this launch's stage and placement
 };
 
-bool ens_supported(int nq) { return nq == 2 || nq == 4 || nq == 6 || nq == 8 || nq == 12; }
+// Nq 2..13: an element's Nq^2 lines fit one 192-thread workgroup
+bool ens_supported(int nq) { return nq >= 2 && nq <= 13; }
 
 // Elements per item (one workgroup): Nq^2 lanes per element, 24, 12, 5, 3
 // and 1 elements for Nq 2, 4, 6, 8, 12.
@@ -77,10 +78,17 @@ size_t ens_lds_nq(bool) {
 size_t ens_item_lds(int nq, bool cv) {
   switch (nq) {
   case 2: return ens_lds_nq<2>(cv);
+  case 3: return ens_lds_nq<3>(cv);
   case 4: return ens_lds_nq<4>(cv);
+  case 5: return ens_lds_nq<5>(cv);
   case 6: return ens_lds_nq<6>(cv);
+  case 7: return ens_lds_nq<7>(cv);
   case 8: return ens_lds_nq<8>(cv);
-  default: return ens_lds_nq<12>(cv);
+  case 9: return ens_lds_nq<9>(cv);
+  case 10: return ens_lds_nq<10>(cv);
+  case 11: return ens_lds_nq<11>(cv);
+  case 12: return ens_lds_nq<12>(cv);
+  default: return ens_lds_nq<13>(cv);
   }
 }
 
@@ -140,10 +148,17 @@ __global__ void __launch_bounds__(kEnsThreads, 3) ensemble_kernel(const EnsArgs 
   const int first = A.cls_item[c], count = A.cls_item[c + 1] - first;
   switch (A.cls_nq[c]) {
   case 2: ens_items<2, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
+  case 3: ens_items<3, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
   case 4: ens_items<4, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
+  case 5: ens_items<5, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
   case 6: ens_items<6, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
+  case 7: ens_items<7, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
   case 8: ens_items<8, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
-  default: ens_items<12, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
+  case 9: ens_items<9, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
+  case 10: ens_items<10, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
+  case 11: ens_items<11, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
+  case 12: ens_items<12, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
+  default: ens_items<13, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
   }
 }
 
@@ -373,8 +388,8 @@ int dgk_ensemble_create(dgk_ensemble **out, int n, const int32_t *p, const int32
         return fail(DGK_ERR_ARG, "instance %d: domain length %g is not positive and finite", i,
                     dom[3 * i + d]);
     if (!ens_supported(p[i] + 1))
-      return fail(DGK_ERR_ORDER, "instance %d: the ensemble runs PolyOrder 1, 3, 5, 7 or 11 "
-                                 "(got %d)", i, p[i]);
+      return fail(DGK_ERR_ORDER, "instance %d: the ensemble runs PolyOrder 1..12 (got %d)", i,
+                  p[i]);
   }
   dgk_ensemble *s = new dgk_ensemble;
   s->n = n;
