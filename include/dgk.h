@@ -259,7 +259,7 @@ int dgk_divlike_dirXYZ_var5_batched(int Nq, int nelem, const double *Mat,
  * initial state and dt, advanced together: one persistent launch per RK
  * stage over all instances, the three stages of a step captured in one
  * hipGraph.  Every instance gives the same bits as a dgk_create_structured
- * solver with the same flags (OPT1 summation order).  PolyOrder 1..12.
+ * solver with the same flags (OPT1 summation order).  PolyOrder 1..15.
  * dom: n triples of domain lengths, or NULL for unit cubes.
  * operator_dir: the directory of the reference's pN.dat files.  flags:
  * DGK_FLAG_FMA, DGK_FLAG_FACTORED_LIFT, DGK_FLAG_CONST_VELOCITY. */
@@ -287,7 +287,8 @@ int dgk_ensemble_time_stages(dgk_ensemble *s, int nstage, double *ms_per_stage);
  * they finish together.  ms_per_class[number of classes] or NULL. */
 int dgk_ensemble_tune(dgk_ensemble *s, int nrep, double *ms_per_class);
 /* what: 0 total DOF, 1 work items per stage, 2 workgroups, 3 LDS bytes,
- * 4 classes of equal order, 5 tuned (0/1), 16+c Nq of class c,
+ * 4 classes of equal order, 5 tuned (0/1), 6 threads per workgroup (192, or
+ * 256 with an instance of PolyOrder 13..15), 16+c Nq of class c,
  * 32+c workgroups of class c */
 int dgk_ensemble_query(dgk_ensemble *s, int what, int64_t *value);
 

@@ -9,7 +9,9 @@
 // runs items of one class only, so that it is compiled and scheduled as one
 // straight code path.  A workgroup of 192 threads (three wavefronts, each on
 // its own SIMD) runs an item as the pencil stage (one lane per node line) of
-// 24, 12, 5, 3 or 1 elements for Nq = 2, 4, 6, 8, 12.
+// as many elements as fill it: 24, 12, 5, 3 or 1 for Nq = 2, 4, 6, 8, 12.
+// An ensemble with an instance of Nq 14..16 (196..256 lines per element)
+// uses workgroups of 256 threads instead.
 // Within a class the items are split into eight contiguous ranges, one per
 // XCD, so that an instance's neighbouring elements stay in one L2.  The
 // split of the grid between the classes follows measured per-class stage
@@ -34,7 +36,8 @@ using namespace dgk;
 
 namespace {
 
-constexpr int kEnsThreads = 192;
+constexpr int kEnsNarrow = 192;  // Nq 2..13
+constexpr int kEnsWide = 256;    // Nq 2..16
 
 struct EnsInst {
   int nq, op;            // Nq and operator slot
@@ -49,7 +52,7 @@ struct EnsOps {          // per distinct Nq
   const double *D, *Lift, *Drow, *LiftP, *lfac;
 };
 
-constexpr int kEnsMaxClass = 12;  // Nq 2..13
+constexpr int kEnsMaxClass = 15;  // Nq 2..16
 
 struct EnsArgs {
   const EnsInst *inst;
@@ -62,33 +65,37 @@ struct EnsArgs {
   const StageParams *params;        // per instance, for this launch's stage and placement
 };
 
-// Nq 2..13: an element's Nq^2 lines fit one 192-thread workgroup
-bool ens_supported(int nq) { return nq >= 2 && nq <= 13; }
+bool ens_supported(int nq) { return nq >= 2 && nq <= 16; }
+// workgroup size for an ensemble whose largest order has max_nq nodes a line
+int ens_threads(int max_nq) { return max_nq <= 13 ? kEnsNarrow : kEnsWide; }
 
 // Elements per item (one workgroup): Nq^2 lanes per element, 24, 12, 5, 3
 // and 1 elements for Nq 2, 4, 6, 8, 12.
-int ens_item_elems(int nq) { return kEnsThreads / (nq * nq); }
+int ens_item_elems(int nq, int nt) { return nt / (nq * nq); }
 
-// (the ensemble stages q in LDS whatever the velocity)
+// LDS of an item (the ensemble stages q in LDS whatever the velocity)
 template <int NQ>
-size_t ens_lds_nq(bool) {
-  return sizeof(double) * (kEnsThreads / (NQ * NQ)) * PencilShape<NQ>::lean_elem(false);
+size_t ens_lds_nq(int nt) {
+  return sizeof(double) * (nt / (NQ * NQ)) * PencilShape<NQ>::lean_elem(false);
 }
 
-size_t ens_item_lds(int nq, bool cv) {
+size_t ens_item_lds(int nq, int nt) {
   switch (nq) {
-  case 2: return ens_lds_nq<2>(cv);
-  case 3: return ens_lds_nq<3>(cv);
-  case 4: return ens_lds_nq<4>(cv);
-  case 5: return ens_lds_nq<5>(cv);
-  case 6: return ens_lds_nq<6>(cv);
-  case 7: return ens_lds_nq<7>(cv);
-  case 8: return ens_lds_nq<8>(cv);
-  case 9: return ens_lds_nq<9>(cv);
-  case 10: return ens_lds_nq<10>(cv);
-  case 11: return ens_lds_nq<11>(cv);
-  case 12: return ens_lds_nq<12>(cv);
-  default: return ens_lds_nq<13>(cv);
+  case 2: return ens_lds_nq<2>(nt);
+  case 3: return ens_lds_nq<3>(nt);
+  case 4: return ens_lds_nq<4>(nt);
+  case 5: return ens_lds_nq<5>(nt);
+  case 6: return ens_lds_nq<6>(nt);
+  case 7: return ens_lds_nq<7>(nt);
+  case 8: return ens_lds_nq<8>(nt);
+  case 9: return ens_lds_nq<9>(nt);
+  case 10: return ens_lds_nq<10>(nt);
+  case 11: return ens_lds_nq<11>(nt);
+  case 12: return ens_lds_nq<12>(nt);
+  case 13: return ens_lds_nq<13>(nt);
+  case 14: return ens_lds_nq<14>(nt);
+  case 15: return ens_lds_nq<15>(nt);
+  default: return ens_lds_nq<16>(nt);
   }
 }
 
@@ -105,10 +112,11 @@ typedef __attribute__((address_space(4))) const int CInt;
 // workgroup's index among them; lb % 8 is its XCD).  An item is the lean
 // pencil stage of the elements first .. first+EPB-1 of an instance; lanes
 // beyond EPB*Nq^2 only join the barriers.
-template <int NQ, int ORDER, bool FMA, bool FACT, bool CV>
+template <int NT, int NQ, int ORDER, bool FMA, bool FACT, bool CV>
 __device__ __forceinline__ void ens_items(const EnsArgs &A, int first, int count, int lb, int nb,
                                           double *smem) {
-  constexpr int NFP = NQ * NQ, EPB = kEnsThreads / NFP;
+  constexpr int NFP = NQ * NQ, EPB = NT / NFP;
+  static_assert(EPB >= 1, "an element's lines must fit the workgroup");
   constexpr int PER = PencilShape<NQ>::lean_elem(false);  // q staged in LDS
   const int xcd = lb % kNumXcd, slot = lb / kNumXcd, nslot = nb / kNumXcd;
   const int per = (count + kNumXcd - 1) / kNumXcd;
@@ -133,12 +141,13 @@ __device__ __forceinline__ void ens_items(const EnsArgs &A, int first, int count
   }
 }
 
-// At least three wavefronts per SIMD (the second launch bound): every form
-// then fits in 168 VGPRs without spilling (the per-node-velocity FMA form
-// would take 174 and hold two).  Four workgroups of three wavefronts fill
-// the CU's twelve.
-template <int ORDER, bool FMA, bool FACT, bool CV>
-__global__ void __launch_bounds__(kEnsThreads, 3) ensemble_kernel(const EnsArgs A) {
+// 192 threads: at least three wavefronts per SIMD (the second launch bound),
+// every form then fits in 168 VGPRs without spilling (the per-node-velocity
+// FMA form would take 174 and hold two); four workgroups of three wavefronts
+// fill the CU's twelve.  256 threads (Nq up to 16): two wavefronts per SIMD,
+// which the LDS of a Nq = 16 element (78 KB) allows anyway.
+template <int NT, int ORDER, bool FMA, bool FACT, bool CV>
+__global__ void __launch_bounds__(NT, NT == kEnsNarrow ? 3 : 2) ensemble_kernel(const EnsArgs A) {
   extern __shared__ double smem[];
   int c = 0;
   while (c + 1 < A.ncls && (int)blockIdx.x >= A.cls_block[c + 1]) ++c;
@@ -147,33 +156,49 @@ __global__ void __launch_bounds__(kEnsThreads, 3) ensemble_kernel(const EnsArgs 
   const int lb = blockIdx.x - b0;
   const int first = A.cls_item[c], count = A.cls_item[c + 1] - first;
   switch (A.cls_nq[c]) {
-  case 2: ens_items<2, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
-  case 3: ens_items<3, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
-  case 4: ens_items<4, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
-  case 5: ens_items<5, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
-  case 6: ens_items<6, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
-  case 7: ens_items<7, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
-  case 8: ens_items<8, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
-  case 9: ens_items<9, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
-  case 10: ens_items<10, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
-  case 11: ens_items<11, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
-  case 12: ens_items<12, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
-  default: ens_items<13, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
+  case 2: ens_items<NT, 2, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
+  case 3: ens_items<NT, 3, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
+  case 4: ens_items<NT, 4, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
+  case 5: ens_items<NT, 5, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
+  case 6: ens_items<NT, 6, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
+  case 7: ens_items<NT, 7, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
+  case 8: ens_items<NT, 8, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
+  case 9: ens_items<NT, 9, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
+  case 10: ens_items<NT, 10, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
+  case 11: ens_items<NT, 11, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
+  case 12: ens_items<NT, 12, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
+  case 13: ens_items<NT, 13, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
+  default:
+    if constexpr (NT >= 256) {
+      switch (A.cls_nq[c]) {
+      case 14: ens_items<NT, 14, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
+      case 15: ens_items<NT, 15, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
+      default: ens_items<NT, 16, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
+      }
+    }
+    break;
   }
 }
 
 using EnsFn = void (*)(const EnsArgs);
 
-template <int ORDER, bool FMA, bool FACT>
+template <int NT, int ORDER, bool FMA, bool FACT>
 EnsFn pick_ens_cv(bool cv) {
-  return cv ? ensemble_kernel<ORDER, FMA, FACT, true> : ensemble_kernel<ORDER, FMA, FACT, false>;
+  return cv ? ensemble_kernel<NT, ORDER, FMA, FACT, true>
+            : ensemble_kernel<NT, ORDER, FMA, FACT, false>;
 }
-template <int ORDER, bool FMA>
+template <int NT, int ORDER, bool FMA>
 EnsFn pick_ens_fact(bool fact, bool cv) {
-  return fact ? pick_ens_cv<ORDER, FMA, true>(cv) : pick_ens_cv<ORDER, FMA, false>(cv);
+  return fact ? pick_ens_cv<NT, ORDER, FMA, true>(cv) : pick_ens_cv<NT, ORDER, FMA, false>(cv);
 }
-EnsFn pick_ens(bool fma, bool fact, bool cv) {
-  return fma ? pick_ens_fact<kOpt1, true>(fact, cv) : pick_ens_fact<kOpt1, false>(fact, cv);
+template <int NT>
+EnsFn pick_ens_nt(bool fma, bool fact, bool cv) {
+  return fma ? pick_ens_fact<NT, kOpt1, true>(fact, cv)
+             : pick_ens_fact<NT, kOpt1, false>(fact, cv);
+}
+EnsFn pick_ens(int nt, bool fma, bool fact, bool cv) {
+  return nt == kEnsNarrow ? pick_ens_nt<kEnsNarrow>(fma, fact, cv)
+                          : pick_ens_nt<kEnsWide>(fma, fact, cv);
 }
 
 const double kRkA[3] = {0.0, 0.75, 1.0 / 3.0};  // mod_common.f90:29
@@ -214,6 +239,7 @@ struct dgk_ensemble {
   int c0 = 2;   // buffer of q0 (never cur)
   hipStream_t stream = nullptr;
   EnsFn fn = nullptr;
+  int nt = kEnsNarrow;  // threads per workgroup
   size_t lds = 0;
   int grid = 0;
   // one captured step per starting (cur, c0), and the (cur, c0) it ends in
@@ -288,7 +314,7 @@ EnsArgs ens_args(const dgk_ensemble *s, int stage) {
 
 int ens_launch_args(dgk_ensemble *s, const EnsArgs &A) {
   const int grid = A.cls_block[A.ncls];
-  hipLaunchKernelGGL(s->fn, dim3(grid), dim3(kEnsThreads), s->lds, s->stream, A);
+  hipLaunchKernelGGL(s->fn, dim3(grid), dim3(s->nt), s->lds, s->stream, A);
   HIPOK(hipGetLastError());
   return DGK_OK;
 }
@@ -388,8 +414,7 @@ int dgk_ensemble_create(dgk_ensemble **out, int n, const int32_t *p, const int32
         return fail(DGK_ERR_ARG, "instance %d: domain length %g is not positive and finite", i,
                     dom[3 * i + d]);
     if (!ens_supported(p[i] + 1))
-      return fail(DGK_ERR_ORDER, "instance %d: the ensemble runs PolyOrder 1..12 (got %d)", i,
-                  p[i]);
+      return fail(DGK_ERR_ORDER, "instance %d: PolyOrder %d outside 1..15", i, p[i]);
   }
   dgk_ensemble *s = new dgk_ensemble;
   s->n = n;
@@ -471,11 +496,14 @@ int dgk_ensemble_create(dgk_ensemble **out, int n, const int32_t *p, const int32
   for (int i = 0; i < n; ++i) order[i] = i;
   std::stable_sort(order.begin(), order.end(),
                    [&](int a, int b) { return s->inst[a].nq < s->inst[b].nq; });
+  int max_nq = 0;
+  for (int i = 0; i < n; ++i) max_nq = std::max(max_nq, s->inst[i].nq);
+  s->nt = ens_threads(max_nq);
   size_t lds = 0;
   for (int i : order) {
-    const int per = ens_item_elems(s->inst[i].nq);
+    const int per = ens_item_elems(s->inst[i].nq, s->nt);
     for (long long e = 0; e < s->inst[i].ne; e += per) items.push_back(make_int2(i, (int)e));
-    lds = std::max(lds, ens_item_lds(s->inst[i].nq, (flags & DGK_FLAG_CONST_VELOCITY) != 0));
+    lds = std::max(lds, ens_item_lds(s->inst[i].nq, s->nt));
   }
   s->nitems = (int)items.size();
   s->lds = lds;
@@ -513,14 +541,14 @@ int dgk_ensemble_create(dgk_ensemble **out, int n, const int32_t *p, const int32
     rc = fail(DGK_ERR_HIP, "hipMemset failed");
   if (!rc) rc = ens_upload_params(s);
   if (!rc) {
-    s->fn = pick_ens((flags & DGK_FLAG_FMA) != 0, (flags & DGK_FLAG_FACTORED_LIFT) != 0,
+    s->fn = pick_ens(s->nt, (flags & DGK_FLAG_FMA) != 0, (flags & DGK_FLAG_FACTORED_LIFT) != 0,
                      (flags & DGK_FLAG_CONST_VELOCITY) != 0);
     int dev = 0, ncu = 0, per_cu = 0;
     if (hipFuncSetAttribute((const void *)s->fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                             (int)s->lds) != hipSuccess ||
         hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)s->fn, kEnsThreads,
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)s->fn, s->nt,
                                                      s->lds) != hipSuccess)
       rc = fail(DGK_ERR_HIP, "ensemble kernel configuration failed");
     s->slots = std::max(1, per_cu) * ncu / kNumXcd * kNumXcd;
@@ -691,6 +719,7 @@ int dgk_ensemble_query(dgk_ensemble *s, int what, int64_t *value) {
   case 3: *value = (int64_t)s->lds; break;   // LDS bytes per workgroup
   case 4: *value = s->ncls; break;           // classes of equal order
   case 5: *value = s->tuned; break;          // grid split from measured times
+  case 6: *value = s->nt; break;             // threads per workgroup
   default:
     if (what >= 16 && what < 16 + s->ncls) {  // Nq of class c = what-16
       *value = s->cls_nq[what - 16];

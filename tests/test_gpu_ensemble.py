@@ -163,17 +163,19 @@ def test_ensemble_step_graph_vs_run(gpu, small):
     b.close()
 
 
-def test_ensemble_every_order(gpu):
-    """One instance of every supported order (p = 1..12, ragged meshes) in
-    one ensemble: each gives the oracle's bits (OPT1 order) after 2 steps,
-    exact and with the uniform-velocity form."""
-    specs = [dict(p=p, ne=(2 + p % 3, 2 + (p + 1) % 2, 2 + p % 2)) for p in range(1, 13)]
+@pytest.mark.parametrize("pmax,threads", [(12, 192), (15, 256)])
+def test_ensemble_every_order(gpu, pmax, threads):
+    """One instance of every order up to pmax (ragged meshes) in one
+    ensemble: each gives the oracle's bits (OPT1 order) after 2 steps, exact
+    and with the uniform-velocity form.  Orders up to 12 run 192-thread
+    workgroups; an instance of order 13..15 switches them to 256."""
+    specs = [dict(p=p, ne=(2 + p % 3, 2 + (p + 1) % 2, 2 + p % 2)) for p in range(1, pmax + 1)]
     inst = _setup(specs, 99)
     want = [_oracle(s, 2) for s in inst]
     for flags in (0, dgk.FLAG_CONST_VELOCITY):
-        say(f"[ens] every order, flags={flags}")
+        say(f"[ens] every order to p={pmax}, flags={flags}")
         ens = _ensemble(inst, flags)
-        assert ens.query(4) == 12
+        assert ens.query(4) == pmax and ens.query(6) == threads
         ens.run(2)
         for i, s in enumerate(inst):
             assert np.array_equal(ens.get_field(i), want[i]), (s["p"], flags)
@@ -182,7 +184,7 @@ def test_ensemble_every_order(gpu):
 
 def test_ensemble_errors(gpu):
     with pytest.raises(dgk.DGKError) as e:
-        dgk.Ensemble([dict(p=13, ne=(2, 2, 2))])
+        dgk.Ensemble([dict(p=16, ne=(2, 2, 2))])
     assert e.value.code == dgk.ERR_ORDER
     inst = _setup([dict(p=3, ne=(2, 2, 2))], 7)
     ens = dgk.Ensemble(inst)
