@@ -262,7 +262,8 @@ int dgk_divlike_dirXYZ_var5_batched(int Nq, int nelem, const double *Mat,
  * solver with the same flags (OPT1 summation order).  PolyOrder 1..15.
  * dom: n triples of domain lengths, or NULL for unit cubes.
  * operator_dir: the directory of the reference's pN.dat files.  flags:
- * DGK_FLAG_FMA, DGK_FLAG_FACTORED_LIFT, DGK_FLAG_CONST_VELOCITY. */
+ * DGK_FLAG_FMA, DGK_FLAG_FACTORED_LIFT, DGK_FLAG_CONST_VELOCITY,
+ * DGK_FLAG_MIXED (implies the factored lift; not with DGK_FLAG_FMA). */
 typedef struct dgk_ensemble dgk_ensemble;
 int dgk_ensemble_create(dgk_ensemble **out, int n, const int32_t *p,
                         const int32_t *NeX, const int32_t *NeY,

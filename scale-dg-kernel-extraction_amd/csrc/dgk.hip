@@ -668,18 +668,7 @@ int finish_create(dgk_solver *s, const double *D1D, const double *Lift) {
   // FP32 split copies: x = hi + lo, hi = fl32(x), lo = fl32(x - hi)
   {
     std::vector<float> d32(2 * nq2), l32(4 * (size_t)Nq);
-    for (int k = 0; k < Nq; ++k)
-      for (int l = 0; l < Nq; ++l) {
-        const double x = dr[(size_t)k * Nq + l];
-        const float hi = (float)x;
-        d32[(size_t)k * 2 * Nq + l] = hi;
-        d32[(size_t)k * 2 * Nq + Nq + l] = (float)(x - (double)hi);
-      }
-    for (int t = 0; t < 2 * Nq; ++t) {
-      const float hi = (float)lf[t];
-      l32[t] = hi;
-      l32[2 * Nq + t] = (float)(lf[t] - (double)hi);
-    }
+    split_tables(Nq, dr.data(), lf.data(), d32.data(), l32.data());
     HIPOK(hipMemcpy(s->Drow32, d32.data(), d32.size() * sizeof(float), hipMemcpyHostToDevice));
     HIPOK(hipMemcpy(s->lfac32, l32.data(), l32.size() * sizeof(float), hipMemcpyHostToDevice));
     s->ops.Drow32 = s->Drow32;

@@ -50,6 +50,7 @@ struct EnsInst {
 
 struct EnsOps {          // per distinct Nq
   const double *D, *Lift, *Drow, *LiftP, *lfac;
+  const double *Drow32, *lfac32;  // DGK_FLAG_MIXED: FP32 split copies (split_tables)
 };
 
 constexpr int kEnsMaxClass = 15;  // Nq 2..16
@@ -136,7 +137,8 @@ __device__ __forceinline__ void ens_items(const EnsArgs &A, int first, int count
     long long e = first_el + el;
     if (!act || e >= P.Ne) e = -1;
     pencil_element<NQ, ORDER, FMA, FACT, true, CV, true, CParams>(
-        P, O.Drow, O.LiftP, O.lfac, smem + el * PER, pl, e, first_el, act);
+        P, ORDER == kMixed ? O.Drow32 : O.Drow, O.LiftP, ORDER == kMixed ? O.lfac32 : O.lfac,
+        smem + el * PER, pl, e, first_el, act);
     __syncthreads();  // LDS is reused by the next item
   }
 }
@@ -191,14 +193,16 @@ template <int NT, int ORDER, bool FMA>
 EnsFn pick_ens_fact(bool fact, bool cv) {
   return fact ? pick_ens_cv<NT, ORDER, FMA, true>(cv) : pick_ens_cv<NT, ORDER, FMA, false>(cv);
 }
+// mixed: the FP32 split contractions (plain products, factored lift)
 template <int NT>
-EnsFn pick_ens_nt(bool fma, bool fact, bool cv) {
+EnsFn pick_ens_nt(bool mixed, bool fma, bool fact, bool cv) {
+  if (mixed) return pick_ens_cv<NT, kMixed, false, true>(cv);
   return fma ? pick_ens_fact<NT, kOpt1, true>(fact, cv)
              : pick_ens_fact<NT, kOpt1, false>(fact, cv);
 }
-EnsFn pick_ens(int nt, bool fma, bool fact, bool cv) {
-  return nt == kEnsNarrow ? pick_ens_nt<kEnsNarrow>(fma, fact, cv)
-                          : pick_ens_nt<kEnsWide>(fma, fact, cv);
+EnsFn pick_ens(int nt, bool mixed, bool fma, bool fact, bool cv) {
+  return nt == kEnsNarrow ? pick_ens_nt<kEnsNarrow>(mixed, fma, fact, cv)
+                          : pick_ens_nt<kEnsWide>(mixed, fma, fact, cv);
 }
 
 const double kRkA[3] = {0.0, 0.75, 1.0 / 3.0};  // mod_common.f90:29
@@ -403,8 +407,11 @@ int dgk_ensemble_create(dgk_ensemble **out, int n, const int32_t *p, const int32
   *out = nullptr;
   if (type_id != DGK_TYPE_OPT1 && type_id != DGK_TYPE_HIP)
     return fail(DGK_ERR_OPTTYPE, "the ensemble evaluates in OPT1's summation order only");
-  if (flags & ~(DGK_FLAG_FMA | DGK_FLAG_FACTORED_LIFT | DGK_FLAG_CONST_VELOCITY))
+  if (flags & ~(DGK_FLAG_FMA | DGK_FLAG_FACTORED_LIFT | DGK_FLAG_CONST_VELOCITY | DGK_FLAG_MIXED))
     return fail(DGK_ERR_ARG, "flags 0x%x not supported by the ensemble", flags);
+  if ((flags & DGK_FLAG_MIXED) && (flags & DGK_FLAG_FMA))
+    return fail(DGK_ERR_ARG, "DGK_FLAG_MIXED runs plain FP32 products (no DGK_FLAG_FMA)");
+  if (flags & DGK_FLAG_MIXED) flags |= DGK_FLAG_FACTORED_LIFT;  // the split form lifts in 1D
   for (int i = 0; i < n; ++i) {
     int32_t sz[7];
     int rc = dgk_mesh_sizes(nex[i], ney[i], nez[i], p[i], sz);
@@ -446,10 +453,14 @@ int dgk_ensemble_create(dgk_ensemble **out, int n, const int32_t *p, const int32
       lf[t] = L[t + (size_t)Np * 3];
       lf[Nq + t] = L[t + (size_t)Np * 1];
     }
+    // FP32 split copies for DGK_FLAG_MIXED, in buffers of doubles' room
+    std::vector<double> d32(Nq * Nq), l32(2 * Nq);
+    split_tables(Nq, Drow.data(), lf.data(), reinterpret_cast<float *>(d32.data()),
+                 reinterpret_cast<float *>(l32.data()));
     EnsOps o{};
-    const std::vector<double> *src[5] = {&D, &L, &Drow, &LiftP, &lf};
-    const double **dst[5] = {&o.D, &o.Lift, &o.Drow, &o.LiftP, &o.lfac};
-    for (int b = 0; b < 5 && !rc; ++b) {
+    const std::vector<double> *src[7] = {&D, &L, &Drow, &LiftP, &lf, &d32, &l32};
+    const double **dst[7] = {&o.D, &o.Lift, &o.Drow, &o.LiftP, &o.lfac, &o.Drow32, &o.lfac32};
+    for (int b = 0; b < 7 && !rc; ++b) {
       double *d = nullptr;
       if ((rc = ens_alloc(&d, src[b]->size()))) break;
       s->opbufs.push_back(d);
@@ -541,7 +552,8 @@ int dgk_ensemble_create(dgk_ensemble **out, int n, const int32_t *p, const int32
     rc = fail(DGK_ERR_HIP, "hipMemset failed");
   if (!rc) rc = ens_upload_params(s);
   if (!rc) {
-    s->fn = pick_ens(s->nt, (flags & DGK_FLAG_FMA) != 0, (flags & DGK_FLAG_FACTORED_LIFT) != 0,
+    s->fn = pick_ens(s->nt, (flags & DGK_FLAG_MIXED) != 0, (flags & DGK_FLAG_FMA) != 0,
+                     (flags & DGK_FLAG_FACTORED_LIFT) != 0,
                      (flags & DGK_FLAG_CONST_VELOCITY) != 0);
     int dev = 0, ncu = 0, per_cu = 0;
     if (hipFuncSetAttribute((const void *)s->fn, hipFuncAttributeMaxDynamicSharedMemorySize,

@@ -77,6 +77,21 @@ long long neighbour_elem(int f, int i, int j, int k, int NeX, int NeY, int NeZ,
   return c[0] + (long long)c[1] * NeX + (long long)c[2] * NeX * NeY;
 }
 
+void split_tables(int Nq, const double *drow, const double *lf, float *d32, float *l32) {
+  for (int k = 0; k < Nq; ++k)
+    for (int l = 0; l < Nq; ++l) {
+      const double x = drow[(size_t)k * Nq + l];
+      const float hi = (float)x;
+      d32[(size_t)k * 2 * Nq + l] = hi;
+      d32[(size_t)k * 2 * Nq + Nq + l] = (float)(x - (double)hi);
+    }
+  for (int t = 0; t < 2 * Nq; ++t) {
+    const float hi = (float)lf[t];
+    l32[t] = hi;
+    l32[2 * Nq + t] = (float)(lf[t] - (double)hi);
+  }
+}
+
 }  // namespace dgk
 
 using namespace dgk;

@@ -22,4 +22,11 @@ long long neighbour_elem(int f, int i, int j, int k, int NeX, int NeY, int NeZ,
 int fail(int code, const char *fmt, ...);
 void clear_error();
 
+// FP32 split copies of the row-major operator drow (Nq x Nq) and of the
+// factored-lift vectors lf = [l_-, l_+] (2 Nq) for DGK_FLAG_MIXED: x = hi +
+// lo, hi = fl32(x), lo = fl32(x - hi).  Row k of d32 (2 Nq floats) holds
+// D_hi(k,.) then D_lo(k,.); l32 (4 Nq floats) holds lf's hi parts then its lo
+// parts.
+void split_tables(int Nq, const double *drow, const double *lf, float *d32, float *l32);
+
 }  // namespace dgk

@@ -182,6 +182,30 @@ def test_ensemble_every_order(gpu, pmax, threads):
         ens.close()
 
 
+def test_ensemble_mixed_precision(gpu, small):
+    """DGK_FLAG_MIXED in the ensemble (FP32 split contractions and lift):
+    every instance within the mixed bound of the oracle (1e-8 rel-L2 after 2
+    steps, and not FP64's bits), and the instances of order 1, 3, 7 equal to a
+    single mixed-precision solver, whose arithmetic is the same."""
+    say("[ens] mixed precision")
+    ens = _ensemble(small, dgk.FLAG_MIXED)
+    ens.run(2)
+    for i, s in enumerate(small):
+        got = ens.get_field(i)
+        err = rel_l2(got, _oracle(s, 2))
+        assert 1e-14 < err <= 1e-8, (i, s["p"], err)
+        if s["p"] in (1, 3, 7):
+            sv = dgk.Solver.structured(*s["ne"], s["p"], s["D"], s["L"], flags=dgk.FLAG_MIXED)
+            sv.set_fields(s["q"], *(np.full_like(s["q"], c) for c in s["vel"]))
+            sv.run(s["dt"], 2)
+            assert np.array_equal(got, sv.get_field()), i
+            sv.close()
+    ens.close()
+    with pytest.raises(dgk.DGKError) as e:
+        dgk.Ensemble(small, flags=dgk.FLAG_MIXED | dgk.FLAG_FMA)
+    assert e.value.code == dgk.ERR_ARG
+
+
 def test_ensemble_errors(gpu):
     with pytest.raises(dgk.DGKError) as e:
         dgk.Ensemble([dict(p=16, ne=(2, 2, 2))])
