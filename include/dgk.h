@@ -55,7 +55,8 @@ extern "C" {
                                       to ~1e-14 relative, not bit-identical) */
 #define DGK_FLAG_CONST_VELOCITY 0x10 /* u,v,w uniform (checked by
                                       dgk_set_fields): only q and q0 stream,
-                                      24 B per DOF-update instead of 48; same
+                                      21.33 B per DOF-update instead of
+                                      45.33 on average over a step; same
                                       results (pencil kernel only)          */
 
 #define DGK_FLAG_MIXED 0x20         /* directional contractions and lift in
@@ -63,8 +64,9 @@ extern "C" {
                                       operands (FP32 sums of hi*hi and of
                                       hi*lo + lo*hi); face-flux differences,
                                       residual and RK in FP64.  Implies the
-                                      factored lift; one-wavefront pencil
-                                      kernel only (Nq 2, 4, 8), no FMA flag */
+                                      factored lift; pencil kernels only
+                                      (compressed connectivity), no FMA
+                                      flag */
 
 /* Fields addressable through dgk_get_field / dgk_field_ptr. */
 #define DGK_FIELD_Q 0

@@ -272,7 +272,7 @@ bool pencil_nq(int Nq) { return Nq >= 2 && Nq <= 16; }
 // wavefront, lean layout); Nq 6, 12, 16: the group kernel.
 StageKernel pick_pencil(int Nq, int order, bool fma, bool fact, bool nt, bool cv, bool wide) {
   const bool one_wave = Nq == 2 || Nq == 4 || Nq == 8;
-  if (order == kMixed && (!one_wave || wide)) return StageKernel{};
+  if (order == kMixed && one_wave && wide) return StageKernel{};
   if (!one_wave || wide)
     return pick_pencil_group(Nq, order, fma, fact, cv, group_epb(Nq, wide));
   switch (Nq) {
@@ -427,8 +427,8 @@ int configure_launch(dgk_solver *s) {
       s->structured && !(s->flags & DGK_FLAG_NODE_KERNEL) && pencil_nq(s->Nq);
   const int var = current_variant(s);
   if (s->order == kMixed && !use_pencil)
-    return fail(DGK_ERR_ARG, "DGK_FLAG_MIXED needs the one-wavefront pencil kernel (Nq 2, 4 "
-                             "or 8, compressed connectivity) without DGK_FLAG_FMA");
+    return fail(DGK_ERR_ARG, "DGK_FLAG_MIXED needs the pencil kernel (compressed "
+                             "connectivity, no DGK_FLAG_NODE_KERNEL) without DGK_FLAG_FMA");
   if ((s->flags & DGK_FLAG_CONST_VELOCITY) && !use_pencil)
     return fail(DGK_ERR_ARG, "DGK_FLAG_CONST_VELOCITY needs the pencil kernel (compressed "
                              "connectivity, no DGK_FLAG_NODE_KERNEL)");
@@ -439,8 +439,8 @@ int configure_launch(dgk_solver *s) {
   else
     s->sk = pick_stage(s->Nq, s->order, s->fma, s->structured);
   if (!s->sk.kernel && s->order == kMixed)
-    return fail(DGK_ERR_ARG, "DGK_FLAG_MIXED needs the one-wavefront pencil kernel (Nq 2, 4 "
-                             "or 8, compressed connectivity) without DGK_FLAG_FMA");
+    return fail(DGK_ERR_ARG, "DGK_FLAG_MIXED needs the pencil kernel (compressed "
+                             "connectivity, no DGK_FLAG_NODE_KERNEL) without DGK_FLAG_FMA");
   if (!s->sk.kernel) return fail(DGK_ERR_ORDER, "Nq %d not supported", s->Nq);
   if ((s->flags & DGK_FLAG_FACTORED_LIFT) && !use_pencil)
     return fail(DGK_ERR_ARG, "DGK_FLAG_FACTORED_LIFT needs the pencil kernel (compressed "

@@ -17,9 +17,11 @@ constexpr int group_maxth() { return NQ >= 13 ? 256 : (NQ >= 9 ? 320 : 576); }
 template <int NQ, int ORDER, bool FMA, bool FACT, bool CV>
 void launch_group(const StageKernel &k, const StageParams &P, const HostOps &H, const double *,
                   hipStream_t st) {
+  // the FP32 split form reads the split copies of D and of the lift vectors
+  const double *dr = ORDER == kMixed ? H.Drow32 : H.Drow;
+  const double *lf = ORDER == kMixed ? H.lfac32 : H.lfac;
   hipLaunchKernelGGL((pencil_group_kernel<NQ, group_maxth<NQ>(), ORDER, FMA, FACT, true, CV>),
-                     dim3((unsigned)k.grid), dim3(k.nt), k.lds, st, P, H.Drow, H.LiftP, H.lfac,
-                     k.epb);
+                     dim3((unsigned)k.grid), dim3(k.nt), k.lds, st, P, dr, H.LiftP, lf, k.epb);
 }
 
 template <int NQ, int ORDER, bool FMA, bool FACT, bool CV>
@@ -46,6 +48,11 @@ StageKernel pick_fact(bool fact, bool cv, int epb) {
 
 template <int NQ>
 StageKernel pick_nq(int order, bool fma, bool fact, bool cv, int epb) {
+  if (order == kMixed) {  // FP32 split contractions: plain products, factored lift only
+    if (fma || !fact) return StageKernel{};
+    return cv ? make_group<NQ, kMixed, false, true, true>(epb)
+              : make_group<NQ, kMixed, false, true, false>(epb);
+  }
   if (order == kGeneral)
     return fma ? pick_fact<NQ, kGeneral, true>(fact, cv, epb)
                : pick_fact<NQ, kGeneral, false>(fact, cv, epb);

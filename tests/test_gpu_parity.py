@@ -446,7 +446,7 @@ def test_state_errors(gpu):
     s.close()
 
 
-@pytest.mark.parametrize("case", [c for c in CS.RUN_CASES if c[1] + 1 in (2, 4, 8)],
+@pytest.mark.parametrize("case", CS.RUN_CASES,
                          ids=lambda c: CS.case_key(*c[0], c[1], c[2]))
 def test_mixed_precision_error(gpu, case):
     """DGK_FLAG_MIXED (FP32 split contractions and lift, FP64 flux
@@ -471,7 +471,7 @@ def test_mixed_precision_error(gpu, case):
 
 
 def test_mixed_precision_refused_where_not_built(gpu):
-    for p, flags in ((5, dgk.FLAG_MIXED), (7, dgk.FLAG_MIXED | dgk.FLAG_FMA),
+    for p, flags in ((7, dgk.FLAG_MIXED | dgk.FLAG_FMA), (5, dgk.FLAG_MIXED | dgk.FLAG_FMA),
                      (7, dgk.FLAG_MIXED | dgk.FLAG_NODE_KERNEL)):
         _, D, L = dgk.read_operator(p)
         with pytest.raises(dgk.DGKError) as e:
