@@ -294,6 +294,10 @@ int dgk_ensemble_tune(dgk_ensemble *s, int nrep, double *ms_per_class);
  * 256 with an instance of PolyOrder 13..15), 16+c Nq of class c,
  * 32+c workgroups of class c */
 int dgk_ensemble_query(dgk_ensemble *s, int what, int64_t *value);
+/* dgk_diagnostics for instance i: out[8] over its owned state, against the
+ * device field ref (Np,Ne) or NULL; w1D its LGL weights (Nq). */
+int dgk_ensemble_diagnostics(dgk_ensemble *s, int i, const double *w1D, const double *ref,
+                             double *out);
 
 #ifdef __cplusplus
 }

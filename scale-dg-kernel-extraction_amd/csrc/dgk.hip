@@ -173,6 +173,37 @@ __global__ void __launch_bounds__(256) diag_final_kernel(const double *part, int
       return fail(DGK_ERR_HIP, "%s failed: %s", #expr, hipGetErrorString(e_)); \
   } while (0)
 
+namespace dgk {
+
+int device_diagnostics(const double *q, const double *ref, long long n, int Nq,
+                       const double es[3], const double *escale, const double *w1D,
+                       double *scratch, void *stream, double *out) {
+  constexpr int kMaxBlocks = 1024;
+  hipStream_t st = (hipStream_t)stream;
+  double *part = scratch, *w = scratch + kDiag * kMaxBlocks, *res = w + 16;
+  if (Nq > 16) return fail(DGK_ERR_ARG, "Nq %d above 16", Nq);
+  HIPOK(hipMemcpyAsync(w, w1D, Nq * sizeof(double), hipMemcpyHostToDevice, st));
+  DiagArgs A{};
+  A.q = q;
+  A.r = ref;
+  A.w1 = w;
+  A.escale = escale;
+  for (int d = 0; d < 3; ++d) A.es[d] = es[d];
+  A.n = n;
+  A.NpNe = n;
+  A.Nq = Nq;
+  A.Np = Nq * Nq * Nq;
+  const int nb = (int)std::min<long long>(kMaxBlocks, (n + 255) / 256);
+  hipLaunchKernelGGL(diag_partial_kernel, dim3(nb), dim3(256), 0, st, A, part);
+  HIPOK(hipGetLastError());
+  hipLaunchKernelGGL(diag_final_kernel, dim3(1), dim3(256), 0, st, part, nb, res);
+  HIPOK(hipGetLastError());
+  HIPOK(hipMemcpyAsync(out, res, kDiag * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIPOK(hipStreamSynchronize(st));
+  return DGK_OK;
+}
+}  // namespace dgk
+
 namespace {
 
 using StageFn = void (*)(const StageParams);
@@ -1014,31 +1045,14 @@ int dgk_set_variant(dgk_solver *s, int variant) {
 int dgk_diagnostics(dgk_solver *s, const double *w1D, const double *ref, double *out) {
   if (!s || !w1D || !out) return fail(DGK_ERR_ARG, "bad argument");
   if (!s->fields_set) return fail(DGK_ERR_STATE, "dgk_set_fields has not been called");
-  constexpr int kMaxBlocks = 1024;
   if (!s->diag) {
-    int rc = dalloc(&s->diag, (size_t)kDiag * kMaxBlocks + 16 + kDiag);
+    int rc = dalloc(&s->diag, (size_t)kDiagScratch);
     if (rc) return rc;
   }
-  double *part = s->diag, *w = s->diag + kDiag * kMaxBlocks, *res = w + 16;
-  HIPOK(hipMemcpyAsync(w, w1D, s->Nq * sizeof(double), hipMemcpyHostToDevice, s->stream));
-  DiagArgs A{};
-  A.q = s->q[s->cur];
-  A.r = ref;
-  A.w1 = w;
-  A.escale = s->nbr ? nullptr : s->escale;
-  for (int d = 0; d < 3; ++d) A.es[d] = s->nbr ? s->es[d] : 1.0;
-  A.n = (long long)s->Np * s->Ne;
-  A.NpNe = A.n;
-  A.Nq = s->Nq;
-  A.Np = s->Np;
-  const int nb = (int)std::min<long long>(kMaxBlocks, (A.n + 255) / 256);
-  hipLaunchKernelGGL(diag_partial_kernel, dim3(nb), dim3(256), 0, s->stream, A, part);
-  HIPOK(hipGetLastError());
-  hipLaunchKernelGGL(diag_final_kernel, dim3(1), dim3(256), 0, s->stream, part, nb, res);
-  HIPOK(hipGetLastError());
-  HIPOK(hipMemcpyAsync(out, res, kDiag * sizeof(double), hipMemcpyDeviceToHost, s->stream));
-  HIPOK(hipStreamSynchronize(s->stream));
-  return DGK_OK;
+  const double ones[3] = {1.0, 1.0, 1.0};
+  return device_diagnostics(s->q[s->cur], ref, (long long)s->Np * s->Ne, s->Nq,
+                            s->nbr ? s->es : ones, s->nbr ? nullptr : s->escale, w1D, s->diag,
+                            s->stream, out);
 }
 
 int dgk_autotune(dgk_solver *s, double dt, int nrep, int *variant_out, double *ms_out) {

@@ -222,6 +222,7 @@ struct dgk_ensemble {
   double *q[3] = {nullptr, nullptr, nullptr};
   double *u = nullptr, *v = nullptr, *w = nullptr;
   int32_t *nbr = nullptr;
+  double *diag = nullptr;  // dgk_ensemble_diagnostics scratch (kDiagScratch)
   EnsInst *dinst = nullptr;
   EnsOps *dops = nullptr;
   std::vector<EnsOps> hops;    // host copy
@@ -387,7 +388,7 @@ extern "C" {
 int dgk_ensemble_destroy(dgk_ensemble *s) {
   if (!s) return DGK_OK;
   ens_drop_graphs(s);
-  void *bufs[] = {s->q[0], s->q[1], s->q[2], s->u, s->v, s->w, s->nbr, s->dinst, s->dops, s->ditems,
+  void *bufs[] = {s->diag, s->q[0], s->q[1], s->q[2], s->u, s->v, s->w, s->nbr, s->dinst, s->dops, s->ditems,
                   s->dparams};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
@@ -720,6 +721,19 @@ int dgk_ensemble_tune(dgk_ensemble *s, int nrep, double *ms_per_class) {
   ens_partition(s);
   s->tuned = true;
   return DGK_OK;
+}
+
+int dgk_ensemble_diagnostics(dgk_ensemble *s, int i, const double *w1D, const double *ref,
+                             double *out) {
+  if (!s || i < 0 || i >= s->n || !w1D || !out) return fail(DGK_ERR_ARG, "bad argument");
+  if (!s->fields_set) return fail(DGK_ERR_STATE, "no fields set");
+  if (!s->diag) {
+    int rc = ens_alloc(&s->diag, (size_t)kDiagScratch);
+    if (rc) return rc;
+  }
+  const EnsInst &I = s->inst[i];
+  return device_diagnostics(s->q[s->cur] + I.off, ref, I.ne * s->np[i], I.nq, I.es, nullptr,
+                            w1D, s->diag, s->stream, out);
 }
 
 int dgk_ensemble_query(dgk_ensemble *s, int what, int64_t *value) {

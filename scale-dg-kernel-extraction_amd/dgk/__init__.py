@@ -38,6 +38,7 @@ EXPORTS = (
     "dgk_ensemble_set_fields", "dgk_ensemble_get_field", "dgk_ensemble_set_dt",
     "dgk_ensemble_step", "dgk_ensemble_run", "dgk_ensemble_time_stages",
     "dgk_ensemble_tune", "dgk_ensemble_query", "dgk_autotune", "dgk_diagnostics",
+    "dgk_ensemble_diagnostics",
     "dgk_matvec_dir_batched", "dgk_gradlike_dirXYZ_batched", "dgk_divlike_dirXYZ_var5_batched",
     "dgk_mesh_setup_device",
 )
@@ -104,6 +105,7 @@ def lib():
         "dgk_ensemble_time_stages": ([vp, i, vp], i),
         "dgk_ensemble_tune": ([vp, i, vp], i),
         "dgk_ensemble_query": ([vp, i, vp], i),
+        "dgk_ensemble_diagnostics": ([vp, i, vp, vp, vp], i),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -443,6 +445,18 @@ class Ensemble:
         v = C.c_longlong(0)
         _check(lib().dgk_ensemble_query(self._h, what, C.byref(v)))
         return v.value
+
+    def diagnostics(self, i, x1D, ref=None):
+        """Solver.diagnostics for instance i (ref: a device address of a
+        (Np,Ne) field, or None)."""
+        w = lgl_weights(x1D)
+        out = np.zeros(8)
+        _check(lib().dgk_ensemble_diagnostics(self._h, i, _p(w), ref, _p(out)))
+        d = dict(min=out[0], max=out[1], mass=out[2], abs_mass=out[3], sum_sq=out[4])
+        if ref:
+            d.update(rel_l2=float(np.sqrt(out[5] / out[6])) if out[6] > 0 else float("nan"),
+                     max_err=out[7], err_sq=out[5], ref_sq=out[6])
+        return d
 
     @property
     def classes(self):

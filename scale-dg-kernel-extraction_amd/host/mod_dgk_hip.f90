@@ -26,6 +26,7 @@ module mod_dgk_hip
   public :: dgk_autotune, dgk_diagnostics
   public :: dgk_ensemble_create, dgk_ensemble_destroy, dgk_ensemble_set_fields
   public :: dgk_ensemble_get_field, dgk_ensemble_set_dt, dgk_ensemble_run, dgk_ensemble_tune
+  public :: dgk_ensemble_diagnostics
 
   interface
     integer(c_int) function dgk_optr_kernel_setup(evaltype, type_id) bind(C, name="dgk_optr_kernel_setup")
@@ -202,6 +203,17 @@ module mod_dgk_hip
       type(c_ptr), value :: ens
       integer(c_int), value :: nrep
       real(c_double), intent(out) :: ms(*)
+    end function
+
+    ! dgk_diagnostics for instance i (0-based) of the ensemble
+    integer(c_int) function dgk_ensemble_diagnostics(ens, i, w1D, ref, out) &
+         bind(C, name="dgk_ensemble_diagnostics")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ens
+      integer(c_int), value :: i
+      real(c_double), intent(in) :: w1D(*)
+      type(c_ptr), value :: ref
+      real(c_double), intent(out) :: out(8)
     end function
 
     type(c_ptr) function dgk_last_error() bind(C, name="dgk_last_error")

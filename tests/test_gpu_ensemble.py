@@ -206,6 +206,27 @@ def test_ensemble_mixed_precision(gpu, small):
     assert e.value.code == dgk.ERR_ARG
 
 
+def test_ensemble_diagnostics(gpu, small):
+    """Per-instance device diagnostics equal those of a single solver in
+    the same state (same reduction, same order), and the error against that
+    solver's field is zero."""
+    ens = _ensemble(small)
+    ens.run(2)
+    for i, s in enumerate(small):
+        say(f"[ens] diagnostics instance {i}")
+        x1D, _, _ = dgk.read_operator(s["p"])
+        sv = dgk.Solver.structured(*s["ne"], s["p"], s["D"], s["L"])
+        sv.set_fields(s["q"], *(np.full_like(s["q"], c) for c in s["vel"]))
+        sv.run(s["dt"], 2)
+        want = sv.diagnostics(x1D)
+        got = ens.diagnostics(i, x1D, ref=sv.field_ptr())
+        for k in ("min", "max", "mass", "abs_mass", "sum_sq"):
+            assert got[k] == want[k], (i, k)
+        assert got["max_err"] == 0.0 and got["rel_l2"] == 0.0, i
+        sv.close()
+    ens.close()
+
+
 def test_ensemble_errors(gpu):
     with pytest.raises(dgk.DGKError) as e:
         dgk.Ensemble([dict(p=16, ne=(2, 2, 2))])
