@@ -371,6 +371,10 @@ def main():
         extra["c5_16cube_p15"] = single_workload("c5_16cube_p15", ("exact", "factored",
                                                                    "factored_constvel",
                                                                    "fma_factored_constvel"))
+        # the two smaller configurations: the reference's CPU-sized case and
+        # the 32^3 mesh of the headline order
+        extra["c1_8cube_p3"] = single_workload("c1_8cube_p3", ("exact", "factored"))
+        extra["c2_32cube_p7"] = single_workload("c2_32cube_p7", ("exact", "factored"))
         extra["c4_ensemble_64"] = ensemble_workload()
 
     cpu = None
