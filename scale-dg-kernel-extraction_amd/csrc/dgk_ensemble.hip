@@ -454,10 +454,13 @@ int dgk_ensemble_create(dgk_ensemble **out, int n, const int32_t *p, const int32
       lf[t] = L[t + (size_t)Np * 3];
       lf[Nq + t] = L[t + (size_t)Np * 1];
     }
-    // FP32 split copies for DGK_FLAG_MIXED, in buffers of doubles' room
+    // FP32 split copies for DGK_FLAG_MIXED, moved bytewise into buffers of
+    // doubles' room (the device reads them as floats)
+    std::vector<float> d32f(2 * Nq * Nq), l32f(4 * Nq);
+    split_tables(Nq, Drow.data(), lf.data(), d32f.data(), l32f.data());
     std::vector<double> d32(Nq * Nq), l32(2 * Nq);
-    split_tables(Nq, Drow.data(), lf.data(), reinterpret_cast<float *>(d32.data()),
-                 reinterpret_cast<float *>(l32.data()));
+    std::memcpy(d32.data(), d32f.data(), d32f.size() * sizeof(float));
+    std::memcpy(l32.data(), l32f.data(), l32f.size() * sizeof(float));
     EnsOps o{};
     const std::vector<double> *src[7] = {&D, &L, &Drow, &LiftP, &lf, &d32, &l32};
     const double **dst[7] = {&o.D, &o.Lift, &o.Drow, &o.LiftP, &o.lfac, &o.Drow32, &o.lfac32};
