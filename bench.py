@@ -82,24 +82,21 @@ def mass(q, w3, J):
     return float(J * np.dot(w3, q.sum(axis=1))), float(J * np.dot(w3, np.abs(q).sum(axis=1)))
 
 
-def cpu_baseline(threads, nstep):
-    """Times the reference's step loop (compiled Fortran, oracle/_ref) on a
-    32^3-element p=7 sample of the same operator and velocity; falls back to
-    the C restatement (oracle/liboracle.so) where the reference binary is
-    absent."""
+def cpu_baseline(threads, nstep, spec, q):
+    """Times the reference's step loop (compiled Fortran, oracle/_ref) for
+    nstep steps of the workload itself (its mesh, order, velocity and dt),
+    starting from the owned field q(Np,Ne); falls back to the C restatement
+    (oracle/liboracle.so) where the reference binary is absent.  Only the
+    step loop is timed (the reference's own timer), not its mesh setup or
+    the file exchange."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle as O
-    spec = ic.WORKLOADS["c3_64cube_p7"]
-    ne = (32, 32, 32)
-    p = spec["p"]
-    m = dgk.mesh_setup(*ne, p, tables=False)
-    q = ic.initial_field(spec["ic"], m["pos_en"], spec["seed"])
-    del m
-    vel = spec["vel"]
+    ne, p, vel = spec["ne"], spec["p"], spec["vel"]
     u, v, w = (np.full_like(q, c) for c in vel)
     dt = ic.cfl_dt(p, vel, ne)
     dof_updates = q.size * 3 * nstep
-    sample = f"32^3 elements, p=7, {nstep} steps ({dof_updates / 1e6:.1f} M DOF-updates)"
+    sample = (f"{ne[0]}x{ne[1]}x{ne[2]} elements, p={p}, {nstep} steps of the workload "
+              f"({dof_updates / 1e6:.1f} M DOF-updates)")
     if O.ref_available():
         _, _, sec, _ = O.ref_run(*ne, p, q, u, v, w, dt, nstep, "OPT1", threads=threads)
         kind = "reference"
@@ -262,7 +259,7 @@ def main():
     ap.add_argument("--variant", default="factored", choices=sorted(VARIANTS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--cpu-steps", type=int, default=5)
+    ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--no-error", action="store_true",
                     help="skip the nstep error run against the bit-exact variant")
@@ -381,7 +378,7 @@ def main():
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         log(f"cpu baseline on {threads} threads")
-        cpu = cpu_baseline(threads, args.cpu_steps)
+        cpu = cpu_baseline(threads, args.cpu_steps, spec, qf)
         log(f"cpu baseline {cpu['value']:.4f} GDOF-updates/s ({cpu['kind']})")
 
     out = {

@@ -15,7 +15,9 @@ flat layout the C and Fortran code index.
 from __future__ import annotations
 
 import ctypes as C
+import contextlib
 import os
+import shutil
 import subprocess
 import tempfile
 
@@ -218,18 +220,25 @@ def _ref(args, cwd, threads=None):
     return r.stdout
 
 
+@contextlib.contextmanager
 def _workdir():
+    """Scratch directory of one reference run (its namelist, input and output
+    files), removed when the run's results have been read."""
     d = tempfile.mkdtemp(prefix="dgref_")
-    os.symlink(OPERATOR_DIR, os.path.join(d, "operator_data"))
-    return d
+    try:
+        os.symlink(OPERATOR_DIR, os.path.join(d, "operator_data"))
+        yield d
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
 
 
 def ref_mesh(NeX, NeY, NeZ, p):
     """Mesh tables as produced by the reference's mesh_setup."""
-    d = _workdir()
-    out = os.path.join(d, "mesh.bin")
-    _ref(["mesh", NeX, NeY, NeZ, p, out], d)
-    raw = open(out, "rb").read()
+    with _workdir() as d:
+        out = os.path.join(d, "mesh.bin")
+        _ref(["mesh", NeX, NeY, NeZ, p, out], d)
+        with open(out, "rb") as f:
+            raw = f.read()
     hdr = np.frombuffer(raw[:24], dtype=np.int32)
     Nq, Np, NfpTot, Ne, NeA, nhn = (int(v) for v in hdr)
     off = 24
@@ -260,14 +269,14 @@ def ref_mesh(NeX, NeY, NeZ, p):
 def ref_optr(p, tname, ix, iy, iz, fb):
     """Reference divlike + lift over nelem blocks: inputs (Np,nelem) x3 and
     (NfpTot,nelem); returns (ox, oy, oz, lift) each (Np,nelem)."""
-    d = _workdir()
     nelem = ix.shape[1]
-    inf, outf = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
-    with open(inf, "wb") as f:
-        for a in (ix, iy, iz, fb):
-            f.write(np.asfortranarray(a, dtype=np.float64).tobytes(order="F"))
-    _ref(["optr", p, tname, nelem, inf, outf], d)
-    o = np.fromfile(outf, dtype=np.float64).reshape((ix.shape[0], nelem, 4), order="F")
+    with _workdir() as d:
+        inf, outf = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
+        with open(inf, "wb") as f:
+            for a in (ix, iy, iz, fb):
+                f.write(np.asfortranarray(a, dtype=np.float64).tobytes(order="F"))
+        _ref(["optr", p, tname, nelem, inf, outf], d)
+        o = np.fromfile(outf, dtype=np.float64).reshape((ix.shape[0], nelem, 4), order="F")
     return tuple(np.asfortranarray(o[:, :, k]) for k in range(4))
 
 
@@ -275,13 +284,13 @@ def ref_optr2(p, a):
     """Reference dirX, dirY, dirZ and gradlike_dirXYZ (OPT1 module) over the
     blocks a(Np,nelem), with D1D for x and D1D_tr for y and z; returns
     (dx, dy, dz, gx, gy, gz) each (Np,nelem)."""
-    d = _workdir()
     nelem = a.shape[1]
-    inf, outf = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
-    with open(inf, "wb") as f:
-        f.write(np.asfortranarray(a, dtype=np.float64).tobytes(order="F"))
-    _ref(["optr2", p, nelem, inf, outf], d)
-    o = np.fromfile(outf, dtype=np.float64).reshape((a.shape[0], nelem, 6), order="F")
+    with _workdir() as d:
+        inf, outf = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
+        with open(inf, "wb") as f:
+            f.write(np.asfortranarray(a, dtype=np.float64).tobytes(order="F"))
+        _ref(["optr2", p, nelem, inf, outf], d)
+        o = np.fromfile(outf, dtype=np.float64).reshape((a.shape[0], nelem, 6), order="F")
     return tuple(np.asfortranarray(o[:, :, k]) for k in range(6))
 
 
@@ -302,14 +311,14 @@ def ref_run(NeX, NeY, NeZ, p, q, u, v, w, dt, nstep, opt="OPT1", mode="run",
     """Runs the reference step loop (mode "run") or a single tendency
     (mode "tend") from owned fields q,u,v,w of shape (Np,Ne).  Returns
     (q_final, minmax(2,nstep), seconds) for "run" and dqdt(Np,Ne) for "tend"."""
-    d = _workdir()
-    conf, icf, outf = (os.path.join(d, n) for n in ("in.conf", "ic.bin", "out.bin"))
-    write_conf(conf, NeX, NeY, NeZ, p, dt, nstep, opt)
-    with open(icf, "wb") as f:
-        for a in (q, u, v, w):
-            f.write(np.asfortranarray(a, dtype=np.float64).tobytes(order="F"))
-    stdout = _ref([mode, conf, icf, outf], d, threads)
-    raw = np.fromfile(outf, dtype=np.float64)
+    with _workdir() as d:
+        conf, icf, outf = (os.path.join(d, n) for n in ("in.conf", "ic.bin", "out.bin"))
+        write_conf(conf, NeX, NeY, NeZ, p, dt, nstep, opt)
+        with open(icf, "wb") as f:
+            for a in (q, u, v, w):
+                f.write(np.asfortranarray(a, dtype=np.float64).tobytes(order="F"))
+        stdout = _ref([mode, conf, icf, outf], d, threads)
+        raw = np.fromfile(outf, dtype=np.float64)
     Np, Ne = q.shape
     qf = raw[:Np * Ne].reshape((Np, Ne), order="F")
     if mode == "tend":

@@ -80,11 +80,11 @@ def main():
     out["stock"] = {}
     prog = os.path.join(REPO, "oracle", "_ref", "scale-dg_extraction")
     for name, cfg in CS.STOCK_RUNS.items():
-        d = O._workdir()
-        with open(os.path.join(d, "input.conf"), "w") as f:
-            f.write(CS.namelist_text(cfg))
-        r = subprocess.run([prog, "input.conf"], cwd=d, capture_output=True, text=True,
-                           check=True, env=dict(os.environ, OMP_NUM_THREADS="4"))
+        with O._workdir() as d:
+            with open(os.path.join(d, "input.conf"), "w") as f:
+                f.write(CS.namelist_text(cfg))
+            r = subprocess.run([prog, "input.conf"], cwd=d, capture_output=True, text=True,
+                               check=True, env=dict(os.environ, OMP_NUM_THREADS="4"))
         lines = [l for l in r.stdout.splitlines() if l[:8].strip().isdigit()]
         out["stock"][name] = {"lines": lines}
     with open(CS.GOLDEN_JSON, "w") as f:
