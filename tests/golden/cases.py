@@ -59,7 +59,11 @@ def optr_inputs(p):
 # Tendency cases: rough random state and per-node random velocities (every
 # face has VelM != VelP), one advect3d_eq_cal_tend.
 TEND_CASES = [((8, 8, 8), 3), ((4, 3, 2), 7), ((2, 3, 2), 8), ((2, 2, 2), 11),
-              ((2, 1, 2), 15)]
+              ((2, 1, 2), 15),
+              # every other order once (extents of 1: an element is its own
+              # neighbour across that axis)
+              ((3, 2, 2), 1), ((2, 3, 3), 2), ((3, 3, 2), 4), ((2, 2, 3), 5), ((2, 3, 2), 6),
+              ((2, 2, 2), 9), ((2, 2, 1), 10), ((1, 2, 2), 12), ((2, 1, 1), 13), ((1, 1, 2), 14)]
 
 
 def tend_inputs(ne, p):
@@ -82,6 +86,9 @@ RUN_CASES = [
     ((3, 2, 2), 8, 10, (0.5, 1.0, -0.25), "rand"),
     ((2, 2, 3), 11, 5, (-0.3, 0.6, 0.9), "rand"),
     ((2, 2, 2), 15, 3, (0.8, 0.6, -0.4), "rand"),
+    ((3, 3, 3), 1, 30, (0.7, -0.4, 0.2), "bump"),
+    ((3, 2, 3), 5, 10, (-0.6, 0.3, 0.8), "rand"),
+    ((2, 1, 2), 13, 3, (0.4, -0.9, 0.5), "rand"),
 ]
 
 
