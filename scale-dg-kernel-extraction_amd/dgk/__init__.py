@@ -120,11 +120,17 @@ def _check(rc):
         raise DGKError(rc, lib().dgk_last_error().decode(errors="replace"))
 
 
-def _p(a):
+def _p(a, dtype=np.float64):
+    """Address of a host array that the library reads or writes as a
+    contiguous column-major block of dtype (double, or int32_t for indices
+    and sizes).  Other element types and strided views are refused rather
+    than converted: the library takes the bytes as they lie."""
     if a is None:
         return None
-    if not (a.flags.f_contiguous or a.ndim <= 1):
-        raise ValueError("array must be Fortran-contiguous")
+    if a.dtype != dtype:
+        raise TypeError(f"array of {a.dtype} where the library expects {np.dtype(dtype)}")
+    if not (a.flags.f_contiguous if a.ndim > 1 else a.flags.c_contiguous):
+        raise ValueError("array must be contiguous in Fortran order")
     return a.ctypes.data_as(C.c_void_p)
 
 
@@ -147,7 +153,7 @@ def read_operator(p: int, opdir: str = OPERATOR_DIR):
 
 def mesh_sizes(NeX, NeY, NeZ, p):
     s = np.zeros(7, dtype=np.int32)
-    _check(lib().dgk_mesh_sizes(NeX, NeY, NeZ, p, _p(s)))
+    _check(lib().dgk_mesh_sizes(NeX, NeY, NeZ, p, _p(s, np.int32)))
     return dict(zip(("Nq", "Np", "Nfp", "NfpTot", "Ne", "NeA", "NhaloNode"),
                     (int(v) for v in s)))
 
@@ -169,10 +175,10 @@ def mesh_setup(NeX, NeY, NeZ, p, dom=(1.0, 1.0, 1.0), x1D=None, tables=True):
                     Escale=np.zeros((Np, Ne, 3), order="F"),
                     Fscale=np.zeros((Nf, Ne), order="F"))
     _check(lib().dgk_mesh_setup(NeX, NeY, NeZ, p, float(dom[0]), float(dom[1]),
-                                float(dom[2]), _p(x1D), _p(arrs["VMapM"]),
-                                _p(arrs["VMapP"]), _p(arrs["normal_fn"]),
+                                float(dom[2]), _p(x1D), _p(arrs["VMapM"], np.int32),
+                                _p(arrs["VMapP"], np.int32), _p(arrs["normal_fn"]),
                                 _p(arrs["Escale"]), _p(arrs["Fscale"]), _p(pos),
-                                _p(halo)))
+                                _p(halo, np.int32)))
     out.update(arrs)
     out.update(pos_en=pos, halo_src_map=halo, x1D=x1D)
     return out
@@ -191,8 +197,9 @@ class Solver:
         h = C.c_void_p()
         D1D, Lift = np.asfortranarray(D1D), np.asfortranarray(Lift)
         _check(lib().dgk_create(C.byref(h), Nq, Ne, NeA, 6 * Nq * Nq, _p(D1D), _p(Lift),
-                                _p(VMapM), _p(VMapP), _p(normal_fn), _p(Escale),
-                                _p(Fscale), _p(halo_src_map), len(halo_src_map),
+                                _p(VMapM, np.int32), _p(VMapP, np.int32), _p(normal_fn),
+                                _p(Escale), _p(Fscale), _p(halo_src_map, np.int32),
+                                len(halo_src_map),
                                 typeid, flags))
         return cls(h, Nq ** 3, Ne, NeA)
 
@@ -328,8 +335,9 @@ def advect3d_eq_cal_tend(q, u, v, w, D1D, D1D_tr, Lift, VMapM, VMapP, normal_fn,
     halos, shape (Np,NeA)); returns dqdt(Np,Ne)."""
     dqdt = np.zeros((Np, Ne), order="F")
     arrs = [np.asfortranarray(a) for a in (q, u, v, w, D1D, D1D_tr, Lift)]
-    _check(lib().dgk_advect3d_eq_cal_tend(_p(dqdt), *[_p(a) for a in arrs], _p(VMapM),
-                                          _p(VMapP), _p(normal_fn), _p(Escale),
+    _check(lib().dgk_advect3d_eq_cal_tend(_p(dqdt), *[_p(a) for a in arrs],
+                                          _p(VMapM, np.int32), _p(VMapP, np.int32),
+                                          _p(normal_fn), _p(Escale),
                                           _p(Fscale), Nq, Np, NfpTot, Ne, NeA, typeid))
     return dqdt
 
@@ -389,7 +397,8 @@ class Ensemble:
                        dtype=np.float64).reshape(n, 3)
         dom = np.ascontiguousarray(dom).ravel()  # n triples
         h = C.c_void_p()
-        _check(lib().dgk_ensemble_create(C.byref(h), n, _p(p), _p(nex), _p(ney), _p(nez),
+        _check(lib().dgk_ensemble_create(C.byref(h), n, _p(p, np.int32), _p(nex, np.int32),
+                                         _p(ney, np.int32), _p(nez, np.int32),
                                          _p(dom), opdir.encode(), typeid, flags))
         self._h = h
         self.n = n

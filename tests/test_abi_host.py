@@ -108,6 +108,31 @@ def test_create_rejects_bad_tables_before_device_use():
     assert e.value.code == dgk.ERR_OPTTYPE
 
 
+def test_binding_refuses_arrays_the_library_would_misread():
+    """The library reads host arrays as contiguous double / int32_t blocks:
+    the binding refuses other element types and strided views instead of
+    handing over their bytes (int64 index maps, float32 operators, every
+    other element of a field)."""
+    m = dgk.mesh_setup(2, 2, 2, 2)
+    x1D, D, L = dgk.read_operator(2)
+    args = [m["VMapM"], m["VMapP"], m["normal_fn"], m["Escale"], m["Fscale"],
+            m["halo_src_map"]]
+    for k, wrong in ((0, m["VMapM"].astype(np.int64)), (5, m["halo_src_map"].astype(np.int64)),
+                     (2, m["normal_fn"].astype(np.float32))):
+        bad = list(args)
+        bad[k] = wrong
+        with pytest.raises(TypeError):
+            dgk.Solver.from_tables(3, m["Ne"], m["NeA"], D, L, *bad)
+    with pytest.raises(TypeError):
+        dgk.Solver.from_tables(3, m["Ne"], m["NeA"], D.astype(np.float32), L, *args)
+    with pytest.raises(ValueError):
+        dgk.Solver.from_tables(3, m["Ne"], m["NeA"], D, L, *args[:5],
+                               np.repeat(m["halo_src_map"], 2)[::2])
+    for view in (np.zeros((3, 4))[:, ::2], np.zeros((3, 4), order="C")):
+        with pytest.raises(ValueError):
+            dgk._p(view)
+
+
 def test_ensemble_workload_draws():
     """The 64-instance workload is a pure function of its seed: orders from
     {3,5,7,11}, per-axis counts from {8,12,16,24}, velocity components in
