@@ -276,7 +276,7 @@ int dgk_ensemble_destroy(dgk_ensemble *s);
 int dgk_ensemble_set_fields(dgk_ensemble *s, int i, const double *q,
                             const double *u, const double *v, const double *w);
 int dgk_ensemble_get_field(dgk_ensemble *s, int i, double *q);
-/* dt[n]: time step of every instance (main.f90:143) */
+/* dt[n]: time step of every instance (the namelist's dt, main.f90:103-107) */
 int dgk_ensemble_set_dt(dgk_ensemble *s, const double *dt);
 /* one SSP-RK3 step of every instance (graph replay, asynchronous) */
 int dgk_ensemble_step(dgk_ensemble *s);
