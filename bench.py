@@ -19,7 +19,9 @@ bounded sample of the workload.
 
 The line also carries the measured variant's rel-L2 and max error against
 the FP64 oracle's bits after the workload's nstep, and (unless --no-extra)
-the p=15 16^3 workload and the 64-instance ensemble.
+the other workloads (8^3 p=3, 32^3 p=7, 16^3 p=15, the 64-instance ensemble)
+and the components either side of the stage kernel on the workload's mesh
+(device mesh build, diagnostics reductions, batched operator kernels).
 
 --copy-calibration first copies 1 GiB device to device (hipMemcpy's copy
 kernel) three times, so that a rocprofv3 --pmc run of this script also
@@ -62,7 +64,7 @@ VARIANTS = {
     "exact_constvel": dgk.FLAG_CONST_VELOCITY,
     "fma_node": dgk.FLAG_FMA | dgk.FLAG_NODE_KERNEL,
     "fma_factored_constvel": dgk.FLAG_FMA | dgk.FLAG_FACTORED_LIFT | dgk.FLAG_CONST_VELOCITY,
-    # FP32 split contractions (pencil kernel, Nq 2, 4, 8), FP64 state
+    # FP32 split contractions and lift (pencil kernels, every order), FP64 state
     "mixed": dgk.FLAG_MIXED,
 }
 
@@ -162,8 +164,6 @@ def mixed_workload(workload, x1D, D, L, spec, nstage=30):
     """The FP32-split variant (DGK_FLAG_MIXED): stage time and its error
     against the oracle's bits after the workload's nstep."""
     ne, p, vel = spec["ne"], spec["p"], spec["vel"]
-    if p + 1 not in (2, 4, 8):
-        return None
     q = _fields(spec, x1D)
     dt = ic.cfl_dt(p, vel, ne)
     s = dgk.Solver.structured(*ne, p, D, L, flags=VARIANTS["mixed"])
@@ -248,6 +248,178 @@ def ensemble_workload(nstage=30):
     log(f"ensemble: {res['gdof_updates_per_s_kernel']} GDOF-updates/s per stage "
         f"(mixed {res['mixed']['gdof_updates_per_s_kernel']})")
     return res
+
+
+def _hbm_row(us, nbytes, what):
+    gbs = nbytes / (us * 1e-6) / 1e9
+    return {"us": round(us, 1), "algorithmic_bytes": int(nbytes), "bytes": what,
+            "GBps": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+
+
+class _Hip:
+    """The HIP runtime that libdgk.so is linked against, opened by its soname
+    (so the process keeps that one runtime): device buffers, a stream and
+    events for the side measurements."""
+
+    def __init__(self):
+        import ctypes as C
+        self.C = C
+        dgk.lib()
+        self.h = C.CDLL("libamdhip64.so.7")
+        vp, pvp = C.c_void_p, C.POINTER(C.c_void_p)
+        for name, args in (("hipMalloc", [pvp, C.c_size_t]), ("hipFree", [vp]),
+                           ("hipMemset", [vp, C.c_int, C.c_size_t]),
+                           ("hipMemcpy", [vp, vp, C.c_size_t, C.c_int]),
+                           ("hipStreamCreate", [pvp]), ("hipStreamDestroy", [vp]),
+                           ("hipStreamSynchronize", [vp]),
+                           ("hipEventCreate", [pvp]), ("hipEventDestroy", [vp]),
+                           ("hipEventRecord", [vp, vp]), ("hipEventSynchronize", [vp]),
+                           ("hipEventElapsedTime", [C.POINTER(C.c_float), vp, vp])):
+            f = getattr(self.h, name)
+            f.argtypes, f.restype = args, C.c_int
+        self.bufs = []
+        st = vp()
+        self.ok(self.h.hipStreamCreate(C.byref(st)), "hipStreamCreate")
+        self.stream = st.value
+
+    def ok(self, rc, what):
+        if rc:
+            raise RuntimeError(f"{what} returned HIP error {rc}")
+
+    def zeros(self, nbytes):
+        p = self.C.c_void_p()
+        self.ok(self.h.hipMalloc(self.C.byref(p), nbytes), "hipMalloc")
+        self.bufs.append(p.value)
+        self.ok(self.h.hipMemset(p.value, 0, nbytes), "hipMemset")
+        return p.value
+
+    def upload(self, a):
+        a = np.asfortranarray(a, dtype=np.float64)
+        p = self.zeros(a.nbytes)
+        self.ok(self.h.hipMemcpy(p, a.ctypes.data, a.nbytes, 1), "hipMemcpy")
+        return p
+
+    def free(self):
+        for p in self.bufs:
+            self.h.hipFree(p)
+        self.bufs = []
+
+    def close(self):
+        self.free()
+        self.h.hipStreamDestroy(self.stream)
+
+    def time_us(self, fn, nrep):
+        """Mean time of fn() (which launches on self.stream) between two
+        events recorded on that stream."""
+        C = self.C
+        fn()
+        self.ok(self.h.hipStreamSynchronize(self.stream), "hipStreamSynchronize")
+        e0, e1 = C.c_void_p(), C.c_void_p()
+        self.ok(self.h.hipEventCreate(C.byref(e0)), "hipEventCreate")
+        self.ok(self.h.hipEventCreate(C.byref(e1)), "hipEventCreate")
+        self.ok(self.h.hipEventRecord(e0, self.stream), "hipEventRecord")
+        for _ in range(nrep):
+            fn()
+        self.ok(self.h.hipEventRecord(e1, self.stream), "hipEventRecord")
+        self.ok(self.h.hipEventSynchronize(e1), "hipEventSynchronize")
+        ms = C.c_float(0.0)
+        self.ok(self.h.hipEventElapsedTime(C.byref(ms), e0, e1), "hipEventElapsedTime")
+        self.h.hipEventDestroy(e0)
+        self.h.hipEventDestroy(e1)
+        return ms.value / nrep * 1e3
+
+
+def side_rows(spec, q, nrep=5):
+    """The components either side of the stage kernel, on the workload's
+    mesh: the device mesh build (against the host build of the same tables),
+    the diagnostics reductions and the batched OPT1 operator kernels over
+    every element (inputs zero-filled: the kernels' time does not depend on
+    the values).  Each is HBM-bound; the row gives its time per call (HIP
+    events on the stream it is launched on, or the host clock around a call
+    that returns to the host), its algorithmic bytes and their fraction of
+    the 8 TB/s peak."""
+    ne, p, vel = spec["ne"], spec["p"], spec["vel"]
+    x1D, D, L = dgk.read_operator(p)
+    sz = dgk.mesh_sizes(*ne, p)
+    Nq, Np, Nf, Ne, nhn = sz["Nq"], sz["Np"], sz["NfpTot"], sz["Ne"], sz["NhaloNode"]
+    hip = _Hip()
+    sh = hip.stream
+
+    def clock(fn):
+        fn()
+        t0 = time.perf_counter()
+        for _ in range(nrep):
+            fn()
+        return (time.perf_counter() - t0) / nrep * 1e6
+
+    out = {"NeX": ne[0], "PolyOrder": p}
+    # device mesh build: every table of mesh_setup, written once
+    sizes = {"VMapM": 4 * Nf * Ne, "VMapP": 4 * Nf * Ne, "normal_fn": 24 * Nf * Ne,
+             "Escale": 24 * Np * Ne, "Fscale": 8 * Nf * Ne, "pos_en": 24 * Np * Ne,
+             "halo_src_map": 4 * nhn}
+    ptrs = {k: hip.zeros(n) for k, n in sizes.items()}
+    row = _hbm_row(hip.time_us(lambda: dgk.mesh_setup_device(*ne, p, x1D, ptrs, stream=sh),
+                               nrep),
+                   sum(sizes.values()), "tables written: VMapM, VMapP, normal_fn, Escale, "
+                                        "Fscale, pos_en, halo_src_map")
+    hip.free()
+    t0 = time.perf_counter()
+    m = dgk.mesh_setup(*ne, p, x1D=x1D)
+    row["host_build_s"] = round(time.perf_counter() - t0, 2)
+    del m
+    out["mesh_setup_device"] = row
+    log(f"mesh build: device {row['us'] / 1e3:.1f} ms, host {row['host_build_s']} s")
+
+    # diagnostics: min/max/mass/sum q^2, and rel-L2 against a second field
+    s = dgk.Solver.structured(*ne, p, D, L, flags=VARIANTS["factored"])
+    s.set_fields(q, *(np.full_like(q, c) for c in vel))
+    ref = s.field_ptr(dgk.FIELD_U)
+    out["diagnostics"] = _hbm_row(clock(lambda: s.diagnostics(x1D)), 8 * Np * Ne,
+                                  "q read once (8 B per DOF); host clock around the call")
+    out["diagnostics_vs_ref"] = _hbm_row(clock(lambda: s.diagnostics(x1D, ref=ref)),
+                                         16 * Np * Ne, "q and the reference field read once")
+    dt = ic.cfl_dt(p, vel, ne)
+    nstep = 10
+    s.run(dt, 2)
+    t0 = time.perf_counter()
+    s.run(dt, nstep)
+    plain = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    s.run(dt, nstep, minmax=True)
+    with_mm = time.perf_counter() - t0
+    out["step_minmax"] = {"us_per_step_added": round((with_mm - plain) / nstep * 1e6, 1),
+                          "bytes": "q read once per step (8 B per DOF)",
+                          "algorithmic_bytes": 8 * Np * Ne}
+    s.close()
+    log(f"diagnostics: {out['diagnostics']['us']:.0f} us, with ref "
+        f"{out['diagnostics_vs_ref']['us']:.0f} us")
+
+    # batched OPT1 operator kernels over every element of the mesh
+    n = Ne
+    dD, dDt = hip.upload(D), hip.upload(D.T)
+    dL = hip.upload(L)
+    ip = [hip.zeros(8 * Np * n) for _ in range(3)]
+    op = [hip.zeros(8 * Np * n) for _ in range(3)]
+    fb = hip.zeros(8 * Nf * n)
+    out["divlike_dirXYZ"] = _hbm_row(
+        hip.time_us(lambda: dgk.divlike_batched(Nq, n, dgk.TYPE_OPT1, dD, dDt, *ip, *op,
+                                                stream=sh), nrep),
+        48 * Np * n, "3 fields in, 3 out")
+    out["gradlike_dirXYZ"] = _hbm_row(
+        hip.time_us(lambda: dgk.gradlike_batched(Nq, n, dD, dDt, ip[0], *op, stream=sh), nrep),
+        32 * Np * n, "1 field in, 3 out")
+    out["matvec_dirX"] = _hbm_row(
+        hip.time_us(lambda: dgk.matvec_dir_batched(Nq, n, 0, dD, ip[0], op[0], stream=sh),
+                    nrep),
+        16 * Np * n, "1 field in, 1 out")
+    out["lift_hexahedral"] = _hbm_row(
+        hip.time_us(lambda: dgk.lift_batched(Nq, n, dL, fb, op[0], stream=sh), nrep),
+        8 * (Nf + Np) * n, "face fluxes in (NfpTot per element), 1 field out")
+    hip.close()
+    log("operators: " + ", ".join(f"{k} {out[k]['us']:.0f} us" for k in
+                                  ("divlike_dirXYZ", "gradlike_dirXYZ", "matvec_dirX",
+                                   "lift_hexahedral")))
+    return out
 
 
 def main():
@@ -373,6 +545,8 @@ def main():
         extra["c1_8cube_p3"] = single_workload("c1_8cube_p3", ("exact", "factored"))
         extra["c2_32cube_p7"] = single_workload("c2_32cube_p7", ("exact", "factored"))
         extra["c4_ensemble_64"] = ensemble_workload()
+        # the rows either side of the stage kernel, on the same mesh
+        extra["side_rows_" + args.workload] = side_rows(spec, qf)
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -84,8 +84,8 @@ constexpr int kDiag = 8;
 
 struct DiagArgs {
   const double *q, *r;  // r may be null
-  const double *w1;     // LGL weights (Nq)
   const double *escale; // (Np,Ne,3) or null
+  double w1[16];        // LGL weights (Nq <= 16), passed by value
   double es[3];
   long long n, NpNe;
   int Nq, Np;
@@ -130,18 +130,32 @@ __device__ __forceinline__ void diag_init(double (&v)[kDiag]) {
   for (int t = 2; t < kDiag; ++t) v[t] = 0.0;
 }
 
+// Grid-stride over the owned nodes; a thread's node index within its element
+// advances by the stride modulo Np, and the node's weight w_i w_j w_k comes
+// from an LDS table of the Np weights (dynamic LDS, Np doubles), so the loop
+// does no integer division.
 __global__ void __launch_bounds__(256) diag_partial_kernel(const DiagArgs A, double *part) {
+  extern __shared__ double wn[];
+  for (int n = threadIdx.x; n < A.Np; n += blockDim.x) {
+    const int i = n % A.Nq, j = (n / A.Nq) % A.Nq, k = n / (A.Nq * A.Nq);
+    wn[n] = A.w1[i] * A.w1[j] * A.w1[k];
+  }
+  __syncthreads();
   double v[kDiag];
   diag_init(v);
   const double J0 = 1.0 / (A.es[0] * A.es[1] * A.es[2]);
-  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < A.n;
-       t += (long long)gridDim.x * blockDim.x) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  const int dn = (int)(stride % A.Np);
+  long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  int nd = (int)(t % A.Np);
+#pragma unroll 4
+  for (; t < A.n; t += stride) {
     const double x = A.q[t];
-    const int nd = (int)(t % A.Np);
-    const int i = nd % A.Nq, j = (nd / A.Nq) % A.Nq, k = nd / (A.Nq * A.Nq);
     const double J =
         A.escale ? 1.0 / (A.escale[t] * A.escale[t + A.NpNe] * A.escale[t + 2 * A.NpNe]) : J0;
-    const double wj = J * (A.w1[i] * A.w1[j] * A.w1[k]);
+    const double wj = J * wn[nd];
+    nd += dn;
+    if (nd >= A.Np) nd -= A.Np;
     v[0] = fmin(v[0], x);
     v[1] = fmax(v[1], x);
     v[2] += wj * x;
@@ -179,14 +193,14 @@ int device_diagnostics(const double *q, const double *ref, long long n, int Nq,
                        const double es[3], const double *escale, const double *w1D,
                        double *scratch, void *stream, double *out) {
   constexpr int kMaxBlocks = 1024;
+  static_assert(kDiag * kMaxBlocks + kDiag == kDiagScratch, "scratch layout");
   hipStream_t st = (hipStream_t)stream;
-  double *part = scratch, *w = scratch + kDiag * kMaxBlocks, *res = w + 16;
-  if (Nq > 16) return fail(DGK_ERR_ARG, "Nq %d above 16", Nq);
-  HIPOK(hipMemcpyAsync(w, w1D, Nq * sizeof(double), hipMemcpyHostToDevice, st));
+  double *part = scratch, *res = scratch + kDiag * kMaxBlocks;
+  if (Nq < 2 || Nq > 16) return fail(DGK_ERR_ARG, "Nq %d outside 2..16", Nq);
   DiagArgs A{};
   A.q = q;
   A.r = ref;
-  A.w1 = w;
+  for (int i = 0; i < Nq; ++i) A.w1[i] = w1D[i];
   A.escale = escale;
   for (int d = 0; d < 3; ++d) A.es[d] = es[d];
   A.n = n;
@@ -194,7 +208,8 @@ int device_diagnostics(const double *q, const double *ref, long long n, int Nq,
   A.Nq = Nq;
   A.Np = Nq * Nq * Nq;
   const int nb = (int)std::min<long long>(kMaxBlocks, (n + 255) / 256);
-  hipLaunchKernelGGL(diag_partial_kernel, dim3(nb), dim3(256), 0, st, A, part);
+  hipLaunchKernelGGL(diag_partial_kernel, dim3(nb), dim3(256), sizeof(double) * A.Np, st, A,
+                     part);
   HIPOK(hipGetLastError());
   hipLaunchKernelGGL(diag_final_kernel, dim3(1), dim3(256), 0, st, part, nb, res);
   HIPOK(hipGetLastError());

@@ -34,7 +34,7 @@ void split_tables(int Nq, const double *drow, const double *lf, float *d32, floa
 // (Np,Ne,3) if not null), against the device field ref (or null): out[8] as
 // dgk_diagnostics documents.  scratch: kDiagScratch device doubles; stream:
 // the hipStream_t to run on.  Waits for the result.
-constexpr long long kDiagScratch = 8 * 1024 + 16 + 8;
+constexpr long long kDiagScratch = 8 * 1024 + 8;  // 1024 block partials, the result
 int device_diagnostics(const double *q, const double *ref, long long n, int Nq,
                        const double es[3], const double *escale, const double *w1D,
                        double *scratch, void *stream, double *out);

@@ -27,7 +27,7 @@ struct MeshArgs {
   int NeX, NeY, NeZ, Nq, Np, Nfp, NfpTot;
   long long Ne;
   double dl[3];
-  const double *x1D;  // device copy (Nq)
+  double x1D[16];  // LGL nodes (Nq <= 16), passed by value
   int32_t *VMapM, *VMapP, *halo;
   double *nrm, *escale, *fscale, *pos;
 };
@@ -161,16 +161,8 @@ extern "C" int dgk_mesh_setup_device(int NeX, int NeY, int NeZ, int p, double do
   A.fscale = Fscale;
   A.pos = pos_en;
   hipStream_t st = (hipStream_t)stream;
-  double *dx = nullptr;
-  if (pos_en) {
-    HIPOK(hipMalloc((void **)&dx, A.Nq * sizeof(double)));
-    if (hipMemcpyAsync(dx, x1D, A.Nq * sizeof(double), hipMemcpyHostToDevice, st) !=
-        hipSuccess) {
-      (void)hipFree(dx);
-      return fail(DGK_ERR_HIP, "x1D upload failed");
-    }
-  }
-  A.x1D = dx;
+  if (pos_en)
+    for (int i = 0; i < A.Nq; ++i) A.x1D[i] = x1D[i];
   const long long nf = A.Ne * A.NfpTot, nn = A.Ne * A.Np;
   const unsigned gf = (unsigned)std::min<long long>((nf + 255) / 256, 65536);
   const unsigned gn = (unsigned)std::min<long long>((nn + 255) / 256, 65536);
@@ -179,7 +171,6 @@ extern "C" int dgk_mesh_setup_device(int NeX, int NeY, int NeZ, int p, double do
   if (pos_en || Escale) hipLaunchKernelGGL(mesh_node_kernel, dim3(gn), dim3(256), 0, st, A);
   hipError_t e = hipGetLastError();
   if (e == hipSuccess) e = hipStreamSynchronize(st);
-  if (dx) (void)hipFree(dx);
   if (e != hipSuccess) return fail(DGK_ERR_HIP, "mesh kernels: %s", hipGetErrorString(e));
   return DGK_OK;
 }
