@@ -133,7 +133,9 @@ __device__ __forceinline__ void diag_init(double (&v)[kDiag]) {
 // Grid-stride over the owned nodes; a thread's node index within its element
 // advances by the stride modulo Np, and the node's weight w_i w_j w_k comes
 // from an LDS table of the Np weights (dynamic LDS, Np doubles), so the loop
-// does no integer division.
+// does no integer division.  REF: A.r is a reference field (sums 5-7); ESC:
+// per-node Escale (A.escale) instead of the uniform A.es.
+template <bool REF, bool ESC>
 __global__ void __launch_bounds__(256) diag_partial_kernel(const DiagArgs A, double *part) {
   extern __shared__ double wn[];
   for (int n = threadIdx.x; n < A.Np; n += blockDim.x) {
@@ -152,7 +154,7 @@ __global__ void __launch_bounds__(256) diag_partial_kernel(const DiagArgs A, dou
   for (; t < A.n; t += stride) {
     const double x = A.q[t];
     const double J =
-        A.escale ? 1.0 / (A.escale[t] * A.escale[t + A.NpNe] * A.escale[t + 2 * A.NpNe]) : J0;
+        ESC ? 1.0 / (A.escale[t] * A.escale[t + A.NpNe] * A.escale[t + 2 * A.NpNe]) : J0;
     const double wj = J * wn[nd];
     nd += dn;
     if (nd >= A.Np) nd -= A.Np;
@@ -161,7 +163,7 @@ __global__ void __launch_bounds__(256) diag_partial_kernel(const DiagArgs A, dou
     v[2] += wj * x;
     v[3] += wj * fabs(x);
     v[4] += x * x;
-    if (A.r) {
+    if constexpr (REF) {
       const double r = A.r[t], d = x - r;
       v[5] += d * d;
       v[6] += r * r;
@@ -208,8 +210,9 @@ int device_diagnostics(const double *q, const double *ref, long long n, int Nq,
   A.Nq = Nq;
   A.Np = Nq * Nq * Nq;
   const int nb = (int)std::min<long long>(kMaxBlocks, (n + 255) / 256);
-  hipLaunchKernelGGL(diag_partial_kernel, dim3(nb), dim3(256), sizeof(double) * A.Np, st, A,
-                     part);
+  auto kern = ref ? (escale ? diag_partial_kernel<true, true> : diag_partial_kernel<true, false>)
+                 : (escale ? diag_partial_kernel<false, true> : diag_partial_kernel<false, false>);
+  hipLaunchKernelGGL(kern, dim3(nb), dim3(256), sizeof(double) * A.Np, st, A, part);
   HIPOK(hipGetLastError());
   hipLaunchKernelGGL(diag_final_kernel, dim3(1), dim3(256), 0, st, part, nb, res);
   HIPOK(hipGetLastError());
@@ -1218,19 +1221,29 @@ int dgk_lift_hexahedral_batched(int Nq, int nelem, const double *Lift, const dou
   if (nelem < 0) return fail(DGK_ERR_ARG, "negative nelem");
   if (nelem == 0) return DGK_OK;
   if (!Lift || !in || !out) return fail(DGK_ERR_ARG, "null pointer");
-  const size_t lds = sizeof(double) * 6 * (size_t)Nq * Nq;
   hipStream_t st = (hipStream_t)stream;
-  using Fn = void (*)(const double *, const double *, double *);
+  using Fn = void (*)(const double *, const double *, double *, long long);
   Fn fn = nullptr;
-#define DGK_LIFT_CASE(NQ) \
-  case NQ: fn = lift_batched_kernel<NQ>; break;
+  int nt = 0, epb = 1;
+  size_t lds = 0;
+#define DGK_LIFT_CASE(NQ)                                                      \
+  case NQ:                                                                     \
+    fn = lift_batched_kernel<NQ>;                                              \
+    nt = LiftShape<NQ>::NT;                                                    \
+    epb = LiftShape<NQ>::EPB;                                                  \
+    lds = LiftShape<NQ>::LDS;                                                  \
+    break;
   switch (Nq) {
     DGK_LIFT_CASE(2) DGK_LIFT_CASE(3) DGK_LIFT_CASE(4) DGK_LIFT_CASE(5) DGK_LIFT_CASE(6)
     DGK_LIFT_CASE(7) DGK_LIFT_CASE(8) DGK_LIFT_CASE(9) DGK_LIFT_CASE(10) DGK_LIFT_CASE(11)
     DGK_LIFT_CASE(12) DGK_LIFT_CASE(13) DGK_LIFT_CASE(14) DGK_LIFT_CASE(15) DGK_LIFT_CASE(16)
   }
 #undef DGK_LIFT_CASE
-  hipLaunchKernelGGL(fn, dim3(nelem), dim3(256), lds, st, Lift, in, out);
+  // enough workgroups to fill the chip several times over; each then walks
+  // the elements in a grid stride
+  const long long groups = (nelem + epb - 1) / epb;
+  const unsigned grid = (unsigned)std::min<long long>(groups, 4096);
+  hipLaunchKernelGGL(fn, dim3(grid), dim3(nt), lds, st, Lift, in, out, (long long)nelem);
   HIPOK(hipGetLastError());
   return DGK_OK;
 }

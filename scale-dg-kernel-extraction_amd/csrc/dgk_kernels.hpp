@@ -401,25 +401,64 @@ __global__ void __launch_bounds__(256)
   }
 }
 
-// Batched tensorprod_Lift_hexahedral: one element per workgroup.
+// Batched tensorprod_Lift_hexahedral (mod_dg_optr_kernel.f90:262-288):
+// persistent workgroups of NT threads.  Up to 256 nodes an element (Nq <= 6)
+// a workgroup takes EPB = 256/Np elements at a time, one node per thread;
+// above that one element at a time, NPT nodes per thread.  A thread's nodes
+// never change, so their six Lift coefficients stay in registers and the
+// table is read once per workgroup instead of once per element; the
+// elements' face fluxes are staged in LDS.  Same products and order of
+// additions as the reference.
 template <int NQ>
-__global__ void __launch_bounds__(256)
-    lift_batched_kernel(const double *Lift, const double *in, double *out) {
-  constexpr int NP = NQ * NQ * NQ, NFP = NQ * NQ, NFT = 6 * NFP;
+struct LiftShape {
+  static constexpr int NP = NQ * NQ * NQ, NFT = 6 * NQ * NQ;
+  static constexpr int NT = NP <= 256 ? 256 : (NP <= 1024 ? (NP + 63) / 64 * 64 : 1024);
+  static constexpr int EPB = NP <= 256 ? 256 / NP : 1;
+  static constexpr int NPT = NP <= 256 ? 1 : (NP + NT - 1) / NT;
+  static constexpr size_t LDS = sizeof(double) * EPB * NFT;
+};
+
+template <int NQ>
+__global__ void __launch_bounds__(LiftShape<NQ>::NT)
+    lift_batched_kernel(const double *Lift, const double *in, double *out, long long nelem) {
+  using S = LiftShape<NQ>;
+  constexpr int NP = S::NP, NFP = NQ * NQ, NFT = S::NFT, EPB = S::EPB, NPT = S::NPT;
   extern __shared__ double smem[];
-  const long long e = blockIdx.x;
-  for (int t = threadIdx.x; t < NFT; t += blockDim.x) smem[t] = in[e * NFT + t];
-  __syncthreads();
-  for (int n = threadIdx.x; n < NP; n += blockDim.x) {
-    const int i = n % NQ, j = (n / NQ) % NQ, k = n / NFP;
-    const double *fb = smem;
-    double s = Lift[n] * fb[i + k * NQ];
-    s = s + Lift[n + NP] * fb[NFP + j + k * NQ];
-    s = s + Lift[n + 2 * NP] * fb[2 * NFP + i + k * NQ];
-    s = s + Lift[n + 3 * NP] * fb[3 * NFP + j + k * NQ];
-    s = s + Lift[n + 4 * NP] * fb[4 * NFP + i + j * NQ];
-    s = s + Lift[n + 5 * NP] * fb[5 * NFP + i + j * NQ];
-    out[e * NP + n] = s;
+  const int t = threadIdx.x;
+  // this thread's element slot and nodes
+  const int es = NP <= 256 ? t / NP : 0;
+  int nn[NPT];
+  double Lc[NPT][6];
+#pragma unroll
+  for (int r = 0; r < NPT; ++r) {
+    const int n = NP <= 256 ? t % NP : t + r * S::NT;
+    const bool ok = (NP <= 256 ? es < EPB : n < NP);
+    nn[r] = ok ? n : -1;
+#pragma unroll
+    for (int f = 0; f < 6; ++f) Lc[r][f] = ok ? Lift[n + f * NP] : 0.0;
+  }
+  for (long long e0 = (long long)blockIdx.x * EPB; e0 < nelem; e0 += (long long)gridDim.x * EPB) {
+    const long long nf = (nelem - e0 < EPB ? nelem - e0 : EPB) * NFT;
+    for (int u = t; u < nf; u += S::NT) smem[u] = in[e0 * NFT + u];
+    __syncthreads();
+    const long long e = e0 + es;
+    if (e < nelem) {
+      const double *fb = smem + es * NFT;
+#pragma unroll
+      for (int r = 0; r < NPT; ++r) {
+        const int n = nn[r];
+        if (n < 0) continue;
+        const int i = n % NQ, j = (n / NQ) % NQ, k = n / NFP;
+        double s = Lc[r][0] * fb[i + k * NQ];
+        s = s + Lc[r][1] * fb[NFP + j + k * NQ];
+        s = s + Lc[r][2] * fb[2 * NFP + i + k * NQ];
+        s = s + Lc[r][3] * fb[3 * NFP + j + k * NQ];
+        s = s + Lc[r][4] * fb[4 * NFP + i + j * NQ];
+        s = s + Lc[r][5] * fb[5 * NFP + i + j * NQ];
+        out[e * NP + n] = s;
+      }
+    }
+    __syncthreads();
   }
 }
 
