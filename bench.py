@@ -16,6 +16,7 @@ bounded sample of the workload.
     python bench.py [--steps K] [--warmup W] [--workload c3_64cube_p7]
                     [--variant factored|exact|fma|...] [--no-cpu-baseline]
                     [--copy-calibration] [--no-error] [--no-extra]
+                    [--ensemble-only | --side-only]
 
 The line also carries the measured variant's rel-L2 and max error against
 the FP64 oracle's bits after the workload's nstep, and (unless --no-extra)
@@ -439,6 +440,9 @@ def main():
                     help="skip the p=15 and ensemble side measurements")
     ap.add_argument("--ensemble-only", action="store_true",
                     help="measure only the 64-instance ensemble (one JSON line)")
+    ap.add_argument("--side-only", action="store_true",
+                    help="measure only the rows beside the stage kernel on the workload's "
+                         "mesh (one JSON line)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -447,6 +451,11 @@ def main():
 
     if args.ensemble_only:
         print(json.dumps({"workload": "c4_ensemble_64", **ensemble_workload()}), flush=True)
+        return
+    if args.side_only:
+        spec = ic.WORKLOADS[args.workload]
+        q = _fields(spec, dgk.read_operator(spec["p"])[0])
+        print(json.dumps({"workload": args.workload, **side_rows(spec, q)}), flush=True)
         return
 
     if args.copy_calibration:
