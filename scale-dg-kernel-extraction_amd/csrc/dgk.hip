@@ -27,6 +27,8 @@ __global__ void __launch_bounds__(256)
     minmax_partial_kernel(const double *q, long long n, double *part) {
   __shared__ double smn[256], smx[256];
   double mn = __builtin_inf(), mx = -__builtin_inf();
+  // unrolled so that several loads of a thread are in flight at once
+#pragma unroll 4
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
        t += (long long)gridDim.x * blockDim.x) {
     const double x = q[t];
