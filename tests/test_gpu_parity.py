@@ -68,6 +68,52 @@ def test_batched_operators_bit_exact(gpu, golden, p, tname):
         assert CS.digest(from_dev(t, (Np, n))) == g[k]["sha256"], k
 
 
+@pytest.mark.parametrize("p", [1, 3, 5, 7, 12, 15])
+def test_batched_operators_many_elements(gpu, golden, p):
+    """Large batches: the lift walks its elements in at most 4096 persistent
+    workgroups (several elements each at low order), the other operators
+    take one workgroup per element.  With the golden elements repeated past
+    4096 lift workgroups' worth, every element's outputs equal the golden
+    elements' bits."""
+    torch = gpu
+    Nq = p + 1
+    Np, Nft = Nq ** 3, 6 * Nq * Nq
+    epb = 256 // Np if Np <= 256 else 1  # elements per lift workgroup
+    n0 = CS.OPTR_NELEM
+    reps = (4096 * epb + 2048) // n0 + 1
+    n = n0 * reps
+    say(f"[operators many] p={p} nelem={n}")
+    _, D, L = dgk.read_operator(p)
+    ix, iy, iz, fb = CS.optr_inputs(p)
+    tile = lambda a: torch.from_numpy(np.tile(np.asfortranarray(a).ravel(order="F"), reps)).cuda()
+    dD, dDt, dL = to_dev(torch, D), to_dev(torch, np.asfortranarray(D.T)), to_dev(torch, L)
+    dix, diy, diz, dfb = (tile(a) for a in (ix, iy, iz, fb))
+    assert dix.numel() == Np * n and dfb.numel() == Nft * n
+    outs = [torch.zeros(Np * n, dtype=torch.float64, device="cuda") for _ in range(7)]
+    tid = dgk.optr_kernel_setup("OPT1")
+    dgk.divlike_batched(Nq, n, tid, dD.data_ptr(), dDt.data_ptr(), dix.data_ptr(),
+                        diy.data_ptr(), diz.data_ptr(), *(o.data_ptr() for o in outs[:3]))
+    dgk.lift_batched(Nq, n, dL.data_ptr(), dfb.data_ptr(), outs[3].data_ptr())
+    dgk.matvec_dir_batched(Nq, n, 0, dD.data_ptr(), dix.data_ptr(), outs[4].data_ptr())
+    dgk.gradlike_batched(Nq, n, dD.data_ptr(), dDt.data_ptr(), dix.data_ptr(),
+                         outs[5].data_ptr(), outs[6].data_ptr(), outs[4].data_ptr())
+    torch.cuda.synchronize()
+    g, g2 = golden["optr"][CS.case_key(p, "OPT1")], golden["optr2"][CS.case_key(p)]
+    want = [g["x"], g["y"], g["z"], g["lift"], None, g2["gradX"], g2["gradY"]]
+    for o, w in zip(outs, want):
+        if w is None:
+            continue
+        blocks = o.view(reps, n0 * Np)
+        assert torch.equal(blocks, blocks[:1].expand(reps, -1))
+        assert CS.digest(from_dev(blocks[0], (Np, n0))) == w["sha256"]
+    # dirX was overwritten by gradlike's z output: check dirX alone
+    dgk.matvec_dir_batched(Nq, n, 0, dD.data_ptr(), dix.data_ptr(), outs[4].data_ptr())
+    torch.cuda.synchronize()
+    blocks = outs[4].view(reps, n0 * Np)
+    assert torch.equal(blocks, blocks[:1].expand(reps, -1))
+    assert CS.digest(from_dev(blocks[0], (Np, n0))) == g2["dirX"]["sha256"]
+
+
 @pytest.mark.parametrize("p", range(1, 16))
 def test_batched_directional_gradlike_var5(gpu, golden, p):
     """dirX/dirY/dirZ and gradlike x, y bit-exact against the reference;
