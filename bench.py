@@ -251,6 +251,32 @@ def ensemble_workload(nstage=30):
     return res
 
 
+def host_inclusive(spec, flags, q, nstep):
+    """The whole job as a host caller sees it through the C-ABI: upload of
+    q, u, v, w from pageable host arrays (dgk_set_fields), nstep steps
+    (dgk_run) and the download of q (dgk_get_field), by the host clock."""
+    ne, p, vel = spec["ne"], spec["p"], spec["vel"]
+    _, D, L = dgk.read_operator(p)
+    dt = ic.cfl_dt(p, vel, ne)
+    uvw = [np.full_like(q, c) for c in vel]
+    s = dgk.Solver.structured(*ne, p, D, L, flags=flags)
+    s.set_fields(q, *uvw)  # first touch of the device buffers, not timed
+    s.run(dt, 1)
+    t0 = time.perf_counter()
+    s.set_fields(q, *uvw)
+    t1 = time.perf_counter()
+    s.run(dt, nstep)
+    t2 = time.perf_counter()
+    s.get_field()
+    t3 = time.perf_counter()
+    s.close()
+    dof = q.size
+    return {"nstep": nstep, "upload_s": round(t1 - t0, 3), "steps_s": round(t2 - t1, 3),
+            "download_s": round(t3 - t2, 3),
+            "gdof_updates_per_s": round(dof * 3 * nstep / (t3 - t0) / 1e9, 2),
+            "bytes": "q, u, v, w up (pageable host memory) and q down, each (Np,NeA) doubles"}
+
+
 def _hbm_row(us, nbytes, what):
     gbs = nbytes / (us * 1e-6) / 1e9
     return {"us": round(us, 1), "algorithmic_bytes": int(nbytes), "bytes": what,
@@ -541,6 +567,10 @@ def main():
 
     extra = {}
     if not args.no_extra:
+        # the same job with its host transfers (the C-ABI hands over host arrays)
+        extra["host_inclusive_" + args.workload] = host_inclusive(spec, VARIANTS[args.variant],
+                                                                  qf, spec["nstep"])
+        log("host-inclusive: " + json.dumps(extra["host_inclusive_" + args.workload]))
         extra["c3_64cube_p7_mixed"] = mixed_workload(args.workload, x1D, D, L, spec)
         # the same mesh with the uniform velocity held as three scalars, and
         # the FMA form with per-node velocity (each autotuned)
