@@ -17,6 +17,13 @@ using namespace dgk;
 namespace dgk {
 
 // update_halo (mod_mesh.f90:216-218): f(Np*Ne + ib) = f(halo_src_map(ib)).
+// f[0..n) = v (a uniform velocity component, halo slots included).
+__global__ void __launch_bounds__(256) fill_kernel(double *f, long long n, double v) {
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+       t += (long long)gridDim.x * blockDim.x)
+    f[t] = v;
+}
+
 __global__ void halo_gather_kernel(double *f, const int32_t *src, int n, long long base) {
   const long long ib = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (ib < n) f[base + ib] = f[src[ib] - 1];
@@ -907,12 +914,24 @@ int dgk_set_fields(dgk_solver *s, const double *q, const double *u, const double
   s->cur = 0;
   s->c0 = 2;
   HIPOK(hipMemcpyAsync(s->q[0], q, nb, hipMemcpyHostToDevice, s->stream));
-  HIPOK(hipMemcpyAsync(s->u, u, nb, hipMemcpyHostToDevice, s->stream));
-  HIPOK(hipMemcpyAsync(s->v, v, nb, hipMemcpyHostToDevice, s->stream));
-  HIPOK(hipMemcpyAsync(s->w, w, nb, hipMemcpyHostToDevice, s->stream));
   int rc;
-  for (double *f : {s->q[0], s->u, s->v, s->w})
-    if ((rc = launch_halo(s, f))) return rc;
+  if ((rc = launch_halo(s, s->q[0]))) return rc;
+  double *vel_dev[3] = {s->u, s->v, s->w};
+  const double *vel_host[3] = {u, v, w};
+  for (int d = 0; d < 3; ++d) {
+    if (s->flags & DGK_FLAG_CONST_VELOCITY) {
+      // verified uniform above: the device copy (halo included) is the
+      // constant, written on the device instead of uploaded
+      const long long n = (long long)s->Np * s->NeA;
+      const unsigned grid = (unsigned)std::min<long long>((n + 255) / 256, 4096);
+      hipLaunchKernelGGL(fill_kernel, dim3(grid), dim3(256), 0, s->stream, vel_dev[d], n,
+                         s->cvel[d]);
+      HIPOK(hipGetLastError());
+    } else {
+      HIPOK(hipMemcpyAsync(vel_dev[d], vel_host[d], nb, hipMemcpyHostToDevice, s->stream));
+      if ((rc = launch_halo(s, vel_dev[d]))) return rc;
+    }
+  }
   HIPOK(hipStreamSynchronize(s->stream));
   s->fields_set = true;
   return DGK_OK;

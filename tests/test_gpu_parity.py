@@ -411,6 +411,11 @@ def test_constant_velocity_form(gpu, p, factored):
         s.close()
     assert np.array_equal(outs[0], outs[1])
     s = dgk.Solver.structured(*ne, p, D, L, flags=dgk.FLAG_CONST_VELOCITY)
+    # the uniform velocity is written on the device, halo slots included
+    s.set_fields(q, *uvw)
+    for which, c in zip((dgk.FIELD_U, dgk.FIELD_V, dgk.FIELD_W), vel):
+        f = s.get_field(which, owned=False)
+        assert f.shape == (s.Np, s.NeA) and np.all(f == c)
     bad = uvw[1].copy(order="F")
     bad[5, 2] += 1e-3
     with pytest.raises(dgk.DGKError) as e:
