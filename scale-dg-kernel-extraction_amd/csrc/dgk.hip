@@ -420,7 +420,8 @@ struct dgk_solver {
   double *nrm = nullptr, *fscale = nullptr, *escale = nullptr;
   double fs[6] = {0, 0, 0, 0, 0, 0}, es[3] = {0, 0, 0};
   double *part = nullptr, *mm = nullptr;
-  double *diag = nullptr;  // diagnostics: partials (8 x 1024), weights (Nq), result (8)
+  double *diag = nullptr;  // diagnostics scratch: partials (8 x 1024), result (8)
+  int32_t *tile_order = nullptr;  // tiled element order (StageParams::tile_order)
   int mm_cap = 0;
   int cur = 0;  // buffer of q
   int c0 = 2;   // buffer of q0 (never cur)
@@ -480,6 +481,9 @@ int pencil_variants(const dgk_solver *s) {
   return (s->Nq >= 13 || s->order == kMixed) ? kPencilWide : kPencilVariants;
 }
 
+template <class T>
+int upload(T **dst, const T *src, size_t n, hipStream_t st);
+
 int configure_launch(dgk_solver *s) {
   const bool use_pencil =
       s->structured && !(s->flags & DGK_FLAG_NODE_KERNEL) && pencil_nq(s->Nq);
@@ -513,6 +517,23 @@ int configure_launch(dgk_solver *s) {
       long long tseq;
       tile_shape(s, &tx, &ty, &tseq);
       s->sk.grid = (tseq + s->sk.epb - 1) / s->sk.epb * kNumXcd;
+      if (!s->tile_order) {
+        // the tiled order, evaluated once on the host (it depends on the
+        // mesh extents only)
+        StageParams T{};
+        T.tnx = s->NeX;
+        T.tny = s->NeY;
+        T.tnz = s->NeZ;
+        T.ttx = tx;
+        T.tty = ty;
+        T.tseq = tseq;
+        std::vector<int32_t> tab((size_t)kNumXcd * tseq);
+        for (int x = 0; x < kNumXcd; ++x)
+          for (long long q = 0; q < tseq; ++q)
+            tab[(size_t)x * tseq + q] = (int32_t)tiled_element(T, x, q);
+        int rc = upload(&s->tile_order, tab.data(), tab.size(), s->stream);
+        if (rc) return rc;
+      }
     } else {
       s->sk.grid = (ngroups + kNumXcd - 1) / kNumXcd * kNumXcd;
     }
@@ -561,6 +582,7 @@ StageParams make_params(dgk_solver *s, int mode, double a, double b, double dt) 
     P.tny = s->NeY;
     P.tnz = s->NeZ;
     tile_shape(s, &P.ttx, &P.tty, &P.tseq);
+    P.tile_order = s->tile_order;
   }
   P.qin = s->q[s->cur];
   P.qout = s->q[3 - s->cur - s->c0];
@@ -756,7 +778,7 @@ int dgk_destroy(dgk_solver *s) {
   drop_graphs(s);
   void *bufs[] = {s->q[0], s->q[1], s->q[2], s->u, s->v, s->w, s->dqdt, s->D, s->Lift, s->lfac, s->Drow, s->LiftP,
                   s->nbr, s->vmapM, s->vmapP, s->halo, s->nrm, s->fscale, s->escale,
-                  s->part, s->mm, s->diag, s->Drow32, s->lfac32};
+                  s->part, s->mm, s->diag, s->Drow32, s->lfac32, s->tile_order};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   if (s->stream) (void)hipStreamDestroy(s->stream);

@@ -768,6 +768,8 @@ __global__ void __launch_bounds__(64)
   long long e, e_any;
   if (P.tnx) {
     if (g * EPW >= P.tseq) return;  // uniform over the workgroup
+    // the mapping's divisions, evaluated here: a table load would head each
+    // wavefront's dependency chain (measured 1-2 % slower at p=7)
     e = tiled_element(P, xcd, g * EPW + el);
     // any valid element of this workgroup, for the address math of idle lanes
     e_any = e;
@@ -807,9 +809,9 @@ __global__ void __launch_bounds__(MAXTH)
   long long e, e_any;
   if (P.tnx) {
     if (g * epb >= P.tseq) return;  // uniform over the workgroup
-    e = act ? tiled_element(P, xcd, g * epb + el) : -1;
+    e = act ? tiled_lookup(P, xcd, g * epb + el) : -1;
     e_any = -1;
-    for (int t = 0; t < epb && e_any < 0; ++t) e_any = tiled_element(P, xcd, g * epb + t);
+    for (int t = 0; t < epb && e_any < 0; ++t) e_any = tiled_lookup(P, xcd, g * epb + t);
     if (e_any < 0) return;  // uniform: no element in this workgroup
   } else {
     const long long ngroups = (P.Ne + epb - 1) / epb;
