@@ -60,6 +60,7 @@ constexpr size_t pencil_group_lds(int epb, bool cv) {
   return sizeof(double) * epb * PencilShape<NQ>::lean_elem(cv);
 }
 
+// Padded LDS position of node n: one pad slot per x-line of Nq nodes.
 template <int NQ>
 __device__ __forceinline__ int pidx(int n) { return n + n / NQ; }
 
@@ -485,6 +486,23 @@ __device__ __forceinline__ void pencil_compute(PT &P, const double *__restrict__
   static_assert(!QG || LEAN, "q lines from memory need the lean layout");
   constexpr int NVOL = LEAN ? (QG ? 1 : 2) : 3;
   const int la = pl % NQ, lb = pl / NQ;
+  // Padded LDS positions of this lane's lines.  For Nq not a power of two
+  // pidx's division would be evaluated at every access, so the positions
+  // take their closed forms (l, i, k < NQ):
+  //   z-line (la,lb,k): pidx(pl + NFP k) = zb + (NFP + NQ) k
+  //   x-line (l,la,lb): pidx(l + NQ pl) = xb + l
+  //   y-line (la,l,lb): pidx(la + NQ l + NFP lb) = yb + (NQ + 1) l
+  //   face slots: pidx(pl) = zb, pidx(i + NQ k) = i + (NQ + 1) k
+  // For a power of two the division is a shift and pidx itself needs fewer
+  // live registers (Nq = 16: 166 against 171 VGPRs, three wavefronts per
+  // SIMD instead of two).
+  constexpr bool kShift = (NQ & (NQ - 1)) == 0;
+  const int zb = pl + lb, xb = (NQ + 1) * pl, yb = la + (NFP + NQ) * lb;
+  auto pz = [&](int k) { return kShift ? pidx<NQ>(pl + NFP * k) : zb + (NFP + NQ) * k; };
+  auto px = [&](int l) { return kShift ? pidx<NQ>(l + NQ * pl) : xb + l; };
+  auto py = [&](int l) { return kShift ? pidx<NQ>(la + NQ * l + NFP * lb) : yb + (NQ + 1) * l; };
+  auto pf = [&](int i, int k) { return kShift ? pidx<NQ>(i + NQ * k) : i + (NQ + 1) * k; };
+  const int ps = kShift ? pidx<NQ>(pl) : zb;
   double *Bq = blk;
   double *Bu = Bq + PAD;  // u, then D_x (LEAN: the partial sum S)
   double *Bv = Bu + PAD;  // v, then D_y
@@ -524,7 +542,7 @@ __device__ __forceinline__ void pencil_compute(PT &P, const double *__restrict__
     if (act) {
 #pragma unroll
       for (int k = 0; k < NQ; ++k) {
-        const int s = pidx<NQ>(pl + NFP * k);
+        const int s = pz(k);
         Bq[s] = q[k];
         if constexpr (!CV && !LEAN) {
           Bu[s] = u[k];
@@ -545,17 +563,17 @@ __device__ __forceinline__ void pencil_compute(PT &P, const double *__restrict__
       for (int l = 0; l < NQ; ++l) ux[l] = CV ? P.cvel[0] : P.u[ebase + l + NQ * pl];
 #pragma unroll
       for (int l = 0; l < NQ; ++l) {
-        qx[l] = QG ? P.qin[ebase + l + NQ * pl] : Bq[pidx<NQ>(l + NQ * pl)];
+        qx[l] = QG ? P.qin[ebase + l + NQ * pl] : Bq[px(l)];
         fx[l] = qx[l] * ux[l];
       }
       sweep<NQ, ORDER, FMA, true>(Drow, fx, dx);
       const double f3 = upwind_flux(qx[0], qP3, -ux[0], -uP3, P.fs[3]);
       const double f1 = upwind_flux(qx[NQ - 1], qP1, ux[NQ - 1], uP1, P.fs[1]);
       if (act) {
-        Bf[2 * FPAD + pidx<NQ>(pl)] = f3;
-        Bf[3 * FPAD + pidx<NQ>(pl)] = f1;
+        Bf[2 * FPAD + ps] = f3;
+        Bf[3 * FPAD + ps] = f1;
 #pragma unroll
-        for (int i = 0; i < NQ; ++i) BS[pidx<NQ>(i + NQ * pl)] = e1s * dx[i];
+        for (int i = 0; i < NQ; ++i) BS[px(i)] = e1s * dx[i];
       }
     }
     __syncthreads();
@@ -567,18 +585,18 @@ __device__ __forceinline__ void pencil_compute(PT &P, const double *__restrict__
       for (int l = 0; l < NQ; ++l) vy[l] = CV ? P.cvel[1] : P.v[ebase + la + NQ * l + NFP * lb];
 #pragma unroll
       for (int l = 0; l < NQ; ++l) {
-        qy[l] = QG ? P.qin[ebase + la + NQ * l + NFP * lb] : Bq[pidx<NQ>(la + NQ * l + NFP * lb)];
+        qy[l] = QG ? P.qin[ebase + la + NQ * l + NFP * lb] : Bq[py(l)];
         fy[l] = qy[l] * vy[l];
       }
       sweep<NQ, ORDER, FMA, false>(Drow, fy, dy);
       const double f0 = upwind_flux(qy[0], qP0, -vy[0], -vP0, P.fs[0]);
       const double f2 = upwind_flux(qy[NQ - 1], qP2, vy[NQ - 1], vP2, P.fs[2]);
       if (act) {
-        Bf[0 * FPAD + pidx<NQ>(pl)] = f0;
-        Bf[1 * FPAD + pidx<NQ>(pl)] = f2;
+        Bf[0 * FPAD + ps] = f0;
+        Bf[1 * FPAD + ps] = f2;
 #pragma unroll
         for (int j = 0; j < NQ; ++j) {
-          const int s = pidx<NQ>(la + NQ * j + NFP * lb);
+          const int s = py(j);
           BS[s] = madd<FMA>(e2s, dy[j], BS[s]);
         }
       }
@@ -590,7 +608,7 @@ __device__ __forceinline__ void pencil_compute(PT &P, const double *__restrict__
       double qy[NQ], vy[NQ], fy[NQ], dy[NQ];
 #pragma unroll
       for (int l = 0; l < NQ; ++l) {
-        const int s = pidx<NQ>(la + NQ * l + NFP * lb);
+        const int s = py(l);
         qy[l] = Bq[s];
         vy[l] = CV ? P.cvel[1] : Bv[s];
         fy[l] = qy[l] * vy[l];
@@ -599,10 +617,10 @@ __device__ __forceinline__ void pencil_compute(PT &P, const double *__restrict__
       const double f0 = upwind_flux(qy[0], qP0, -vy[0], -vP0, P.fs[0]);
       const double f2 = upwind_flux(qy[NQ - 1], qP2, vy[NQ - 1], vP2, P.fs[2]);
       if (act) {
-        Bf[0 * FPAD + pidx<NQ>(pl)] = f0;
-        Bf[1 * FPAD + pidx<NQ>(pl)] = f2;
+        Bf[0 * FPAD + ps] = f0;
+        Bf[1 * FPAD + ps] = f2;
 #pragma unroll
-        for (int j = 0; j < NQ; ++j) Bv[pidx<NQ>(la + NQ * j + NFP * lb)] = dy[j];
+        for (int j = 0; j < NQ; ++j) Bv[py(j)] = dy[j];
       }
     }
 
@@ -611,7 +629,7 @@ __device__ __forceinline__ void pencil_compute(PT &P, const double *__restrict__
       double qx[NQ], ux[NQ], fx[NQ], dx[NQ];
 #pragma unroll
       for (int l = 0; l < NQ; ++l) {
-        const int s = pidx<NQ>(l + NQ * pl);
+        const int s = px(l);
         qx[l] = Bq[s];
         ux[l] = CV ? P.cvel[0] : Bu[s];
         fx[l] = qx[l] * ux[l];
@@ -620,10 +638,10 @@ __device__ __forceinline__ void pencil_compute(PT &P, const double *__restrict__
       const double f3 = upwind_flux(qx[0], qP3, -ux[0], -uP3, P.fs[3]);
       const double f1 = upwind_flux(qx[NQ - 1], qP1, ux[NQ - 1], uP1, P.fs[1]);
       if (act) {
-        Bf[2 * FPAD + pidx<NQ>(pl)] = f3;
-        Bf[3 * FPAD + pidx<NQ>(pl)] = f1;
+        Bf[2 * FPAD + ps] = f3;
+        Bf[3 * FPAD + ps] = f1;
 #pragma unroll
-        for (int i = 0; i < NQ; ++i) Bu[pidx<NQ>(i + NQ * pl)] = dx[i];
+        for (int i = 0; i < NQ; ++i) Bu[px(i)] = dx[i];
       }
     }
     __syncthreads();
@@ -636,16 +654,16 @@ __device__ __forceinline__ void pencil_compute(PT &P, const double *__restrict__
 #pragma unroll
     for (int k = 0; k < NQ; ++k) q0[k] = need_q0 ? ld_stream(P.q0 + base + NFP * k) : 0.0;
 #pragma unroll
-    for (int k = 0; k < NQ; ++k) q[k] = QG ? P.qin[base + NFP * k] : Bq[pidx<NQ>(pl + NFP * k)];
+    for (int k = 0; k < NQ; ++k) q[k] = QG ? P.qin[base + NFP * k] : Bq[pz(k)];
     z_phase(q);
   }
 #pragma unroll
   for (int k = 0; k < NQ; ++k) {
-    const int s = pidx<NQ>(pl + NFP * k);
-    const double f0 = Bf[0 * FPAD + pidx<NQ>(i + NQ * k)];  // in(i,k,1)
-    const double f2 = Bf[1 * FPAD + pidx<NQ>(i + NQ * k)];  // in(i,k,3)
-    const double f3 = Bf[2 * FPAD + pidx<NQ>(j + NQ * k)];  // in(j,k,4)
-    const double f1 = Bf[3 * FPAD + pidx<NQ>(j + NQ * k)];  // in(j,k,2)
+    const int s = pz(k);
+    const double f0 = Bf[0 * FPAD + pf(i, k)];  // in(i,k,1)
+    const double f2 = Bf[1 * FPAD + pf(i, k)];  // in(i,k,3)
+    const double f3 = Bf[2 * FPAD + pf(j, k)];  // in(j,k,4)
+    const double f1 = Bf[3 * FPAD + pf(j, k)];  // in(j,k,2)
     double L[6];
     if constexpr (ORDER == kMixed) {
       // lfac holds the split vectors as FP32: [l_-hi, l_+hi, l_-lo, l_+lo]
