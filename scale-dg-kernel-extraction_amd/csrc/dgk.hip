@@ -436,16 +436,27 @@ struct dgk_solver {
 
 namespace {
 
-// Built-in default tuning variant (overridden by dgk_set_variant):
-// non-temporal streaming stores measured fastest at Nq = 8 (64^3 elements),
-// and with uniform velocity there also the linear element order (the
-// autotuner's pick, 5-10 % faster than the tiled one); at Nq = 2 (64-byte
-// elements, 16 per wavefront) plain stores, 17 % faster.
+// Built-in default tuning variant (overridden by dgk_set_variant), from
+// sweeps of every variant at 16^3-64^3: the linear element order with
+// non-temporal stores, except
+// - the tiled order at Nq = 3, and at Nq = 8 and 12 with per-node velocity
+//   (p=7, 64^3: 1230 against 1335 us; p=11: 2-4 % faster), where the
+//   neighbours' face traces are the larger share of the traffic;
+// - plain stores at Nq = 2 (64-byte elements, 16 per wavefront) and at
+//   Nq = 4 with uniform velocity (p=3, 64^3: 90.6 against 98.4 us).
+// Linear against tiled elsewhere: p=1 6-15 %, p=3 6-19 %, p=4 3 %,
+// p=5-15 0-4 % faster.
 int default_pencil_variant(const dgk_solver *s) {
-  if (s->Nq == 2) return 0;
-  if (s->Nq == 8 && (s->flags & DGK_FLAG_CONST_VELOCITY) && s->order != kMixed)
-    return kPencilNtStore | kPencilLinearOrder;
-  return kPencilNtStore;
+  const bool cv = (s->flags & DGK_FLAG_CONST_VELOCITY) != 0;
+  switch (s->Nq) {
+  case 2: return kPencilLinearOrder;
+  case 4: return cv ? kPencilLinearOrder : kPencilNtStore | kPencilLinearOrder;
+  case 3: return kPencilNtStore;
+  case 8:
+  case 12:
+    return cv && s->order != kMixed ? kPencilNtStore | kPencilLinearOrder : kPencilNtStore;
+  default: return kPencilNtStore | kPencilLinearOrder;
+  }
 }
 
 int current_variant(const dgk_solver *s) {
