@@ -439,9 +439,10 @@ namespace {
 // Built-in default tuning variant (overridden by dgk_set_variant), from
 // sweeps of every variant at 16^3-64^3: the linear element order with
 // non-temporal stores, except
-// - the tiled order at Nq = 3, and at Nq = 8 and 12 with per-node velocity
-//   (p=7, 64^3: 1230 against 1335 us; p=11: 2-4 % faster), where the
-//   neighbours' face traces are the larger share of the traffic;
+// - the tiled order at Nq = 3 and 12, and at Nq = 8 with per-node velocity
+//   (p=7, 64^3: 1230 against 1335 us; p=11: 2-4 % faster with the factored
+//   lift and with uniform velocity), where the neighbours' face traces are
+//   the larger share of the traffic;
 // - plain stores at Nq = 2 (64-byte elements, 16 per wavefront) and at
 //   Nq = 4 with uniform velocity (p=3, 64^3: 90.6 against 98.4 us).
 // Linear against tiled elsewhere: p=1 6-15 %, p=3 6-19 %, p=4 3 %,
@@ -451,10 +452,9 @@ int default_pencil_variant(const dgk_solver *s) {
   switch (s->Nq) {
   case 2: return kPencilLinearOrder;
   case 4: return cv ? kPencilLinearOrder : kPencilNtStore | kPencilLinearOrder;
-  case 3: return kPencilNtStore;
-  case 8:
-  case 12:
-    return cv && s->order != kMixed ? kPencilNtStore | kPencilLinearOrder : kPencilNtStore;
+  case 3:
+  case 12: return kPencilNtStore;
+  case 8: return cv && s->order != kMixed ? kPencilNtStore | kPencilLinearOrder : kPencilNtStore;
   default: return kPencilNtStore | kPencilLinearOrder;
   }
 }
