@@ -162,27 +162,59 @@ def error_vs_oracle(workload, flags, x1D, D, L, spec):
 
 
 def mixed_workload(workload, x1D, D, L, spec, nstage=30):
-    """The FP32-split variant (DGK_FLAG_MIXED): stage time and its error
-    against the oracle's bits after the workload's nstep."""
+    """DGK_FLAG_MIXED (FP32 contractions of the lines' remainders after
+    their cubic part, FP64 elsewhere) on the workload: stage time with
+    per-node and with uniform velocity, and its error against the
+    bit-identical FP64 variant run alongside on the device, after every step
+    of the workload's nstep (rel-L2, max error, mass drift)."""
     ne, p, vel = spec["ne"], spec["p"], spec["vel"]
+    nstep = spec["nstep"]
     q = _fields(spec, x1D)
     dt = ic.cfl_dt(p, vel, ne)
-    s = dgk.Solver.structured(*ne, p, D, L, flags=VARIANTS["mixed"])
-    s.set_fields(q, *(np.full_like(q, c) for c in vel))
-    del q
-    s.run(dt, 1)
-    ms = s.time_stages(dt, nstage)
-    s.close()
-    err = error_vs_oracle(workload, VARIANTS["mixed"], x1D, D, L, spec)
-    dof = ne[0] * ne[1] * ne[2] * (p + 1) ** 3
-    res = {"gdof_updates_per_s": round(dof / (ms * 1e-3) / 1e9, 2),
-           "kernel_us": round(ms * 1e3, 1),
-           "hbm_frac": round(BYTES_PER_DOF * dof / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-           "error_vs_fp64_oracle": err,
-           "variant": "FP32 split contractions and lift (hi*hi and hi*lo+lo*hi FP32 sums), "
-                      "FP64 flux differences, residual and RK"}
-    log(f"mixed: {res['gdof_updates_per_s']} GDOF-updates/s, rel-L2 {err['rel_l2']:.2e} after "
-        f"{err['nstep']} steps")
+    uvw = [np.full_like(q, c) for c in vel]
+    res = {}
+    for name, flags in (("per_node_velocity", VARIANTS["mixed"]),
+                        ("uniform_velocity", VARIANTS["mixed"] | dgk.FLAG_CONST_VELOCITY)):
+        s = dgk.Solver.structured(*ne, p, D, L, flags=flags)
+        s.set_fields(q, *uvw)
+        s.run(dt, 1)
+        ms = s.time_stages(dt, nstage)
+        s.close()
+        bpd = BYTES_PER_DOF_CV if flags & dgk.FLAG_CONST_VELOCITY else BYTES_PER_DOF
+        dof = ne[0] * ne[1] * ne[2] * (p + 1) ** 3
+        res[name] = {"gdof_updates_per_s": round(dof / (ms * 1e-3) / 1e9, 2),
+                     "kernel_us": round(ms * 1e3, 1),
+                     "hbm_frac": round(bpd * dof / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    a = dgk.Solver.structured(*ne, p, D, L, flags=VARIANTS["mixed"])
+    b = dgk.Solver.structured(*ne, p, D, L, flags=VARIANTS["exact"])
+    for x in (a, b):
+        x.set_fields(q, *uvw)
+    del q, uvw
+    m0 = a.diagnostics(x1D)
+    per_step = []
+    for _ in range(nstep):
+        a.step(dt)
+        b.step(dt)
+        a.synchronize()
+        b.synchronize()
+        per_step.append(float(f"{a.diagnostics(x1D, ref=b)['rel_l2']:.4e}"))
+    d = a.diagnostics(x1D, ref=b)
+    db = b.diagnostics(x1D)
+    a.close()
+    b.close()
+    res["error_vs_fp64_oracle"] = {
+        "nstep": nstep, "rel_l2": d["rel_l2"], "max_abs": d["max_err"],
+        "max_rel": d["max_err"] / max(abs(db["min"]), abs(db["max"])),
+        "mass_drift_rel": abs(d["mass"] - m0["mass"]) / m0["abs_mass"],
+        "rel_l2_per_step": per_step,
+        "reference": "exact variant (bit-identical to the FP64 oracle), stepped alongside"}
+    res["variant"] = ("FP32 contractions (operator split hi+lo) of each line's remainder after "
+                      "its cubic part, which is differentiated in FP64; lines whose remainder "
+                      "exceeds 1/16 of the line in FP64; lift, flux differences, residual and "
+                      "RK in FP64")
+    log(f"mixed: {res['per_node_velocity']['gdof_updates_per_s']} GDOF-updates/s "
+        f"(uniform velocity {res['uniform_velocity']['gdof_updates_per_s']}), rel-L2 "
+        f"{d['rel_l2']:.2e} after {nstep} steps")
     return res
 
 
@@ -212,8 +244,9 @@ def single_workload(workload, variants, nstage=30):
 def ensemble_workload(nstage=30):
     """The 64-instance ensemble (orders 3, 5, 7, 11, uneven meshes, uniform
     per-instance velocity): one persistent launch per stage, grid split
-    tuned per order class; FP64 with fused multiply-adds, and the FP32-split
-    mixed form."""
+    tuned per order class; FP64 with fused multiply-adds, and the mixed
+    form with its error against the FP64 ensemble after the workload's 50
+    steps."""
     inst = ic.ensemble_workload(**{k: v for k, v in ic.ENSEMBLE.items() if k in ("n", "seed")})
     fields = []
     for sp in inst:
@@ -244,8 +277,29 @@ def ensemble_workload(nstage=30):
     ens = build(dgk.FLAG_CONST_VELOCITY | dgk.FLAG_MIXED)
     ms = ens.time_stages(nstage)
     res["mixed"] = {"gdof_updates_per_s_kernel": round(dof / (ms * 1e-3) / 1e9, 2),
-                    "variant": "FP32 split contractions and lift, FP64 state, uniform velocity"}
+                    "variant": "FP32 contractions of the lines' remainders after their cubic "
+                               "part, FP64 elsewhere, uniform velocity"}
     ens.close()
+    # mixed against the bit-identical FP64 ensemble after the workload's
+    # nstep, per instance
+    nstep = ic.ENSEMBLE["nstep"]
+    a = build(dgk.FLAG_CONST_VELOCITY | dgk.FLAG_MIXED)
+    b = build(dgk.FLAG_CONST_VELOCITY)
+    a.run(nstep)
+    b.run(nstep)
+    errs, num, den = [], 0.0, 0.0
+    for i in range(len(inst)):
+        x, y = b.get_field(i), a.get_field(i)
+        d2, r2 = float(np.sum((y - x) ** 2)), float(np.sum(x * x))
+        errs.append((d2 / r2) ** 0.5)
+        num += d2
+        den += r2
+    a.close()
+    b.close()
+    res["mixed"]["error_vs_fp64"] = {
+        "nstep": nstep, "max_instance_rel_l2": max(errs),
+        "median_instance_rel_l2": float(np.median(errs)), "ensemble_rel_l2": (num / den) ** 0.5,
+        "reference": "the exact FP64 ensemble (bit-identical per instance to the oracle)"}
     log(f"ensemble: {res['gdof_updates_per_s_kernel']} GDOF-updates/s per stage "
         f"(mixed {res['mixed']['gdof_updates_per_s_kernel']})")
     return res

@@ -59,10 +59,13 @@ extern "C" {
                                       45.33 on average over a step; same
                                       results (pencil kernel only)          */
 
-#define DGK_FLAG_MIXED 0x20         /* directional contractions and lift in
-                                      FP32 with hi/lo split operators and
-                                      operands (FP32 sums of hi*hi and of
-                                      hi*lo + lo*hi); face-flux differences,
+#define DGK_FLAG_MIXED 0x20         /* directional contractions in FP32:
+                                      each line's linear part is taken out
+                                      and differentiated in FP64, the
+                                      remainder contracted with the operator
+                                      split into hi + lo FP32 parts (two FP32
+                                      sums); lift in FP32 with split vectors
+                                      and operands; face-flux differences,
                                       residual and RK in FP64.  Implies the
                                       factored lift; pencil kernels only
                                       (compressed connectivity), no FMA

@@ -258,7 +258,7 @@ void launch_pencil(const StageKernel &k, const StageParams &P, const HostOps &H,
                    const double *Lift, hipStream_t st) {
   (void)Lift;
   const double *dr = ORDER == kMixed ? H.Drow32 : H.Drow;
-  const double *lf = ORDER == kMixed ? H.lfac32 : H.lfac;
+  const double *lf = H.lfac;
   hipLaunchKernelGGL((pencil_kernel<NQ, ORDER, FMA, FACT, NT, CV>), dim3((unsigned)k.grid),
                      dim3(64), k.lds, st, P, dr, H.LiftP, lf);
 }
@@ -415,7 +415,7 @@ struct dgk_solver {
   double *q[3] = {nullptr, nullptr, nullptr};
   double *u = nullptr, *v = nullptr, *w = nullptr, *dqdt = nullptr;
   double *D = nullptr, *Lift = nullptr, *lfac = nullptr, *Drow = nullptr, *LiftP = nullptr;
-  double *Drow32 = nullptr, *lfac32 = nullptr;  // FP32 split operators (mixed precision)
+  double *Drow32 = nullptr;  // FP32 split operator and line-fit tables (mixed precision)
   int32_t *nbr = nullptr, *vmapM = nullptr, *vmapP = nullptr, *halo = nullptr;
   double *nrm = nullptr, *fscale = nullptr, *escale = nullptr;
   double fs[6] = {0, 0, 0, 0, 0, 0}, es[3] = {0, 0, 0};
@@ -725,8 +725,7 @@ int finish_create(dgk_solver *s, const double *D1D, const double *Lift) {
   const size_t nq2 = (size_t)Nq * Nq, nlift = (size_t)6 * Np;
   if ((rc = dalloc(&s->D, nq2)) || (rc = dalloc(&s->Lift, nlift)) ||
       (rc = dalloc(&s->lfac, 2 * (size_t)Nq)) || (rc = dalloc(&s->Drow, nq2)) ||
-      (rc = dalloc(&s->LiftP, nlift)) || (rc = dalloc(&s->Drow32, nq2)) ||
-      (rc = dalloc(&s->lfac32, 2 * (size_t)Nq)))
+      (rc = dalloc(&s->LiftP, nlift)) || (rc = dalloc(&s->Drow32, nq2 + 3 * Nq)))
     return rc;
   // pair-packed Lift for the pencil kernel: line pl = i + Nq*j holds the
   // 6*Nq values v = 6k + f, Lift(i,j,k,f); pair m of line pl at
@@ -756,14 +755,13 @@ int finish_create(dgk_solver *s, const double *D1D, const double *Lift) {
   }
   HIPOK(hipMemcpy(s->lfac, lf.data(), lf.size() * sizeof(double), hipMemcpyHostToDevice));
   s->ops.lfac = s->lfac;
-  // FP32 split copies: x = hi + lo, hi = fl32(x), lo = fl32(x - hi)
+  // FP32 split copies (x = hi + lo, hi = fl32(x), lo = fl32(x - hi)) and the
+  // node tables of the line fit
   {
-    std::vector<float> d32(2 * nq2), l32(4 * (size_t)Nq);
-    split_tables(Nq, dr.data(), lf.data(), d32.data(), l32.data());
-    HIPOK(hipMemcpy(s->Drow32, d32.data(), d32.size() * sizeof(float), hipMemcpyHostToDevice));
-    HIPOK(hipMemcpy(s->lfac32, l32.data(), l32.size() * sizeof(float), hipMemcpyHostToDevice));
+    std::vector<double> d32(nq2 + 3 * Nq);
+    split_tables(Nq, dr.data(), d32.data());
+    HIPOK(hipMemcpy(s->Drow32, d32.data(), d32.size() * sizeof(double), hipMemcpyHostToDevice));
     s->ops.Drow32 = s->Drow32;
-    s->ops.lfac32 = s->lfac32;
   }
   if ((rc = alloc_fields(s))) return rc;
   if ((rc = configure_launch(s))) return rc;
@@ -789,7 +787,7 @@ int dgk_destroy(dgk_solver *s) {
   drop_graphs(s);
   void *bufs[] = {s->q[0], s->q[1], s->q[2], s->u, s->v, s->w, s->dqdt, s->D, s->Lift, s->lfac, s->Drow, s->LiftP,
                   s->nbr, s->vmapM, s->vmapP, s->halo, s->nrm, s->fscale, s->escale,
-                  s->part, s->mm, s->diag, s->Drow32, s->lfac32, s->tile_order};
+                  s->part, s->mm, s->diag, s->Drow32, s->tile_order};
   for (void *b : bufs)
     if (b) (void)hipFree(b);
   if (s->stream) (void)hipStreamDestroy(s->stream);

@@ -50,7 +50,7 @@ struct EnsInst {
 
 struct EnsOps {          // per distinct Nq
   const double *D, *Lift, *Drow, *LiftP, *lfac;
-  const double *Drow32, *lfac32;  // DGK_FLAG_MIXED: FP32 split copies (split_tables)
+  const double *Drow32;  // DGK_FLAG_MIXED: FP32 split rows and line-fit tables (split_tables)
 };
 
 constexpr int kEnsMaxClass = 15;  // Nq 2..16
@@ -137,7 +137,7 @@ __device__ __forceinline__ void ens_items(const EnsArgs &A, int first, int count
     long long e = first_el + el;
     if (!act || e >= P.Ne) e = -1;
     pencil_element<NQ, ORDER, FMA, FACT, true, CV, true, CParams>(
-        P, ORDER == kMixed ? O.Drow32 : O.Drow, O.LiftP, ORDER == kMixed ? O.lfac32 : O.lfac,
+        P, ORDER == kMixed ? O.Drow32 : O.Drow, O.LiftP, O.lfac,
         smem + el * PER, pl, e, first_el, act);
     __syncthreads();  // LDS is reused by the next item
   }
@@ -454,17 +454,14 @@ int dgk_ensemble_create(dgk_ensemble **out, int n, const int32_t *p, const int32
       lf[t] = L[t + (size_t)Np * 3];
       lf[Nq + t] = L[t + (size_t)Np * 1];
     }
-    // FP32 split copies for DGK_FLAG_MIXED, moved bytewise into buffers of
-    // doubles' room (the device reads them as floats)
-    std::vector<float> d32f(2 * Nq * Nq), l32f(4 * Nq);
-    split_tables(Nq, Drow.data(), lf.data(), d32f.data(), l32f.data());
-    std::vector<double> d32(Nq * Nq), l32(2 * Nq);
-    std::memcpy(d32.data(), d32f.data(), d32f.size() * sizeof(float));
-    std::memcpy(l32.data(), l32f.data(), l32f.size() * sizeof(float));
+    // FP32 split copies and line-fit nodes for DGK_FLAG_MIXED, in buffers of
+    // doubles' room (the device reads the split parts as floats)
+    std::vector<double> d32(Nq * Nq + 3 * Nq);
+    split_tables(Nq, Drow.data(), d32.data());
     EnsOps o{};
-    const std::vector<double> *src[7] = {&D, &L, &Drow, &LiftP, &lf, &d32, &l32};
-    const double **dst[7] = {&o.D, &o.Lift, &o.Drow, &o.LiftP, &o.lfac, &o.Drow32, &o.lfac32};
-    for (int b = 0; b < 7 && !rc; ++b) {
+    const std::vector<double> *src[6] = {&D, &L, &Drow, &LiftP, &lf, &d32};
+    const double **dst[6] = {&o.D, &o.Lift, &o.Drow, &o.LiftP, &o.lfac, &o.Drow32};
+    for (int b = 0; b < 6 && !rc; ++b) {
       double *d = nullptr;
       if ((rc = ens_alloc(&d, src[b]->size()))) break;
       s->opbufs.push_back(d);

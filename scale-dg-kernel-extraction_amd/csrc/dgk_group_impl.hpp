@@ -19,7 +19,7 @@ void launch_group(const StageKernel &k, const StageParams &P, const HostOps &H, 
                   hipStream_t st) {
   // the FP32 split form reads the split copies of D and of the lift vectors
   const double *dr = ORDER == kMixed ? H.Drow32 : H.Drow;
-  const double *lf = ORDER == kMixed ? H.lfac32 : H.lfac;
+  const double *lf = H.lfac;
   hipLaunchKernelGGL((pencil_group_kernel<NQ, group_maxth<NQ>(), ORDER, FMA, FACT, true, CV>),
                      dim3((unsigned)k.grid), dim3(k.nt), k.lds, st, P, dr, H.LiftP, lf, k.epb);
 }

@@ -15,6 +15,7 @@
 
 #include <cerrno>
 #include <cmath>
+#include <cstdint>
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
@@ -77,19 +78,77 @@ long long neighbour_elem(int f, int i, int j, int k, int NeX, int NeY, int NeZ,
   return c[0] + (long long)c[1] * NeX + (long long)c[2] * NeX * NeY;
 }
 
-void split_tables(int Nq, const double *drow, const double *lf, float *d32, float *l32) {
-  for (int k = 0; k < Nq; ++k)
-    for (int l = 0; l < Nq; ++l) {
-      const double x = drow[(size_t)k * Nq + l];
-      const float hi = (float)x;
-      d32[(size_t)k * 2 * Nq + l] = hi;
-      d32[(size_t)k * 2 * Nq + Nq + l] = (float)(x - (double)hi);
+// Legendre-Gauss-Lobatto nodes on [-1,1] (ascending): x_0 = -1, x_{Nq-1} = 1
+// and the roots of P'_{Nq-1} between them, by Newton's iteration on
+// x P_N(x) - P_{N-1}(x), which vanishes at all of them, from the
+// Chebyshev-Gauss-Lobatto points (the method of G. von Winckel's lglnodes).
+void lgl_nodes(int Nq, double *x) {
+  const int N = Nq - 1;
+  for (int j = 0; j < Nq; ++j) {
+    double xj = -std::cos(M_PI * j / N), prev = 2.0;
+    for (int it = 0; it < 100 && std::fabs(xj - prev) > 1e-16; ++it) {
+      double p0 = 1.0, p1 = xj;  // P_{k-1}, P_k
+      for (int k = 2; k <= N; ++k) {
+        const double p2 = ((2 * k - 1) * xj * p1 - (k - 1) * p0) / k;
+        p0 = p1;
+        p1 = p2;
+      }
+      prev = xj;
+      if (j == 0 || j == N) break;
+      xj = xj - (xj * p1 - p0) / (Nq * p1);
     }
-  for (int t = 0; t < 2 * Nq; ++t) {
-    const float hi = (float)lf[t];
-    l32[t] = hi;
-    l32[2 * Nq + t] = (float)(lf[t] - (double)hi);
+    x[j] = xj;
   }
+  x[0] = -1.0;
+  x[N] = 1.0;
+}
+
+// Layout (see dgk_internal.hpp): the line fit of DGK_FLAG_MIXED takes out
+// the polynomial of degree deg = min(3, Nq-1) through the nodes 0, a, b and
+// Nq-1 (a = Nq/3, b = 2Nq/3) in the Newton basis 1, t, p2 = t(t-1),
+// p3 = t(t-1)(t-t_a) of t = (x+1)/2; the remainder vanishes at those nodes,
+// so their slots in the split rows are free and carry the basis
+// derivatives d p2/dx and d p3/dx at the row's node (as two dwords each).
+void split_tables(int Nq, const double *drow, double *d32) {
+  const int a = Nq / 3, b = 2 * Nq / 3;
+  std::vector<double> x(Nq), t(Nq);
+  lgl_nodes(Nq, x.data());
+  for (int l = 0; l < Nq; ++l) t[l] = 0.5 * (x[l] + 1.0);
+  t[0] = 0.0;
+  t[Nq - 1] = 1.0;
+  auto p2 = [&](double s) { return s * (s - 1.0); };
+  auto p3 = [&](double s) { return s * (s - 1.0) * (s - t[a]); };
+  // derivatives with respect to x (dt/dx = 1/2)
+  auto q2 = [&](double s) { return 0.5 * (2.0 * s - 1.0); };
+  auto q3 = [&](double s) { return 0.5 * ((2.0 * s - 1.0) * (s - t[a]) + s * (s - 1.0)); };
+  std::vector<float> rows(2 * (size_t)Nq * Nq);
+  for (int k = 0; k < Nq; ++k) {
+    float *hi = rows.data() + (size_t)k * 2 * Nq, *lo = hi + Nq;
+    for (int l = 0; l < Nq; ++l) {
+      const double v = drow[(size_t)k * Nq + l];
+      hi[l] = (float)v;
+      lo[l] = (float)(v - (double)hi[l]);
+    }
+    uint32_t w2[2], w3[2];
+    const double d2 = q2(t[k]), d3 = q3(t[k]);
+    std::memcpy(w2, &d2, sizeof d2);
+    std::memcpy(w3, &d3, sizeof d3);
+    std::memcpy(&hi[0], &w2[0], 4);
+    std::memcpy(&hi[Nq - 1], &w2[1], 4);
+    std::memcpy(&lo[0], &w3[0], 4);
+    std::memcpy(&lo[Nq - 1], &w3[1], 4);
+  }
+  std::memcpy(d32, rows.data(), rows.size() * sizeof(float));
+  double *T = d32 + (size_t)Nq * Nq, *P2 = T + Nq, *P3 = P2 + Nq;
+  for (int l = 0; l < Nq; ++l) {
+    T[l] = t[l];
+    P2[l] = p2(t[l]);
+    P3[l] = p3(t[l]);
+  }
+  // the end slots of T (t = 0 and 1 there, never read) hold 1/p2(t_a) and
+  // 1/p3(t_b), the Newton divisors
+  T[0] = Nq >= 3 ? 1.0 / p2(t[a]) : 0.0;
+  T[Nq - 1] = Nq >= 4 ? 1.0 / p3(t[b]) : 0.0;
 }
 
 }  // namespace dgk

@@ -22,12 +22,16 @@ long long neighbour_elem(int f, int i, int j, int k, int NeX, int NeY, int NeZ,
 int fail(int code, const char *fmt, ...);
 void clear_error();
 
-// FP32 split copies of the row-major operator drow (Nq x Nq) and of the
-// factored-lift vectors lf = [l_-, l_+] (2 Nq) for DGK_FLAG_MIXED: x = hi +
-// lo, hi = fl32(x), lo = fl32(x - hi).  Row k of d32 (2 Nq floats) holds
-// D_hi(k,.) then D_lo(k,.); l32 (4 Nq floats) holds lf's hi parts then its lo
-// parts.
-void split_tables(int Nq, const double *drow, const double *lf, float *d32, float *l32);
+// Tables of DGK_FLAG_MIXED.  d32 (Nq^2 + 3 Nq doubles' room): row k of the
+// row-major operator drow (Nq x Nq) as 2 Nq floats, D_hi(k,.) then D_lo(k,.)
+// (x = hi + lo, hi = fl32(x), lo = fl32(x - hi)), rows filling the first Nq^2
+// doubles, with the basis derivatives of the line fit packed into the slots
+// of the fit nodes; then three rows of Nq doubles over the LGL nodes x_l:
+// t_l = (x_l + 1)/2 (its end slots holding the fit's two Newton divisors),
+// p2(t_l) and p3(t_l) (dgk_host.cpp gives the basis).
+void split_tables(int Nq, const double *drow, double *d32);
+// Legendre-Gauss-Lobatto nodes x[Nq] on [-1,1], ascending.
+void lgl_nodes(int Nq, double *x);
 
 // The device reductions of dgk_diagnostics over the n = Np Ne owned values
 // of q (elements of Nq^3 nodes; uniform Escale es, or the per-node escale

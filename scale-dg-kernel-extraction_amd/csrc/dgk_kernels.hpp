@@ -18,11 +18,11 @@ constexpr int kOpt1 = 2;     // mod_dg_optr_kernel.f90:42
 // Left-to-right sums at every p (no split): the OPT1 module's dirX/dirY/dirZ
 // and gradlike kernels (mod_dg_optr_kernel_opt1.F90.erb:51-129, 166-219).
 constexpr int kSequential = 3;
-// Mixed precision (pencil kernels only): the directional contractions and the
-// lift in FP32 with every operator entry split into hi + lo FP32 parts and
-// every operand likewise, hi*hi accumulated in one FP32 sum and the
-// hi*lo + lo*hi corrections in another; the face-flux differences, the
-// residual sum and the RK update stay FP64.
+// Mixed precision (pencil kernels only): the directional contractions in
+// FP32 with every operator entry split into hi + lo FP32 parts, applied to
+// each line's remainder after its cubic part (taken out and differentiated
+// in FP64); the face-flux differences, the lift, the residual sum and the RK
+// update stay FP64.
 constexpr int kMixed = 4;
 constexpr int kNumXcd = 8;   // MI355X: 8 XCDs, workgroups dispatched round-robin
 

@@ -14,8 +14,7 @@ struct HostOps {
   const double *Drow;
   const double *lfac;
   const double *LiftP;   // pair-packed Lift for the pencil kernel
-  const double *Drow32;  // mixed precision: per row D_hi then D_lo as FP32 (Nq^2 doubles' room)
-  const double *lfac32;  // mixed precision: [l_-, l_+] hi then lo as FP32 (2 Nq doubles' room)
+  const double *Drow32;  // mixed precision: split rows and line-fit tables (split_tables)
 };
 
 struct StageKernel;

@@ -344,38 +344,122 @@ __device__ __forceinline__ void sgpr_row_split(const double *Drow, int k, double
   }
 }
 
-// x = hi + lo with hi = fl32(x), lo = fl32(x - hi).
-__device__ __forceinline__ void split_f64(double x, float &hi, float &lo) {
-  hi = (float)x;
-  lo = (float)(x - (double)hi);
-}
-
 // Rows of D in flight during a sweep: row k is loaded once output
 // k - kSweepDepth is done (kSweepDepth rows of SGPRs live).  A depth of 2,
 // which overlaps a row's load with the previous output's arithmetic, measured
 // no faster at p = 7, 11, 15 than 1.
 constexpr int kSweepDepth = 1;
 
+// A line of the mixed form whose FP64 remainder after the fit exceeds this
+// fraction of the line's largest value is contracted in FP64.
+constexpr double kMixedRough = 1.0 / 16.0;
+
+// The line fit of the mixed form: the polynomial of degree DEG through the
+// nodes 0, A, B and NQ-1 of a line (Newton basis 1, t, t(t-1),
+// t(t-1)(t-t_A) in t = (x+1)/2) is differentiated exactly in FP64; only the
+// remainder, zero at those nodes, goes through FP32.
+template <int NQ>
+struct LineFit {
+  static constexpr int DEG = NQ >= 4 ? 3 : NQ - 1;
+  static constexpr int A = NQ / 3, B = 2 * NQ / 3;
+  static constexpr bool fit(int l) {
+    return l == 0 || l == NQ - 1 || (DEG >= 2 && l == A) || (DEG >= 3 && l == B);
+  }
+};
+
+__device__ __forceinline__ double dword_pair(float lo, float hi) {
+  return __builtin_bit_cast(double, (unsigned long long)__builtin_bit_cast(unsigned, lo) |
+                                        ((unsigned long long)__builtin_bit_cast(unsigned, hi) << 32));
+}
+
 // out[k] = sum_l D(k,l) f[l] in the reference's order, one SGPR row at a time.
-// ORDER == kMixed: the FP32 split form (Drow then holds the split rows).
+// ORDER == kMixed: the FP32 form, Drow holding the split rows (D_hi(k,.),
+// D_lo(k,.) as FP32) and after them the rows t, p2, p3 of the line fit
+// (split_tables).  f = g + r with g the fit polynomial (LineFit): D g = g'
+// exactly at the nodes (D differentiates polynomials of degree <= p
+// exactly), evaluated in FP64 as (b + c2 p2' + c3 p3')/2; r, zero at the fit
+// nodes and small against f where f is smooth across the element, is
+// rounded to FP32 and contracted as D_hi r + D_lo r in two FP32 sums.  The
+// FP32 rounding then scales with |r| instead of |f|: at 64^3, p=7 on the
+// sine-mode field the contraction error drops ~1000x against hi/lo-split
+// operands of the whole line; at p <= 3 the fit is the whole line and no
+// FP32 arithmetic is left in the sweep.  Lines with a large remainder
+// (kMixedRough) take the FP64 contraction.
 template <int NQ, int ORDER, bool FMA, bool MAT_FIRST>
 __device__ __forceinline__ void sweep(const double *Drow, const double (&f)[NQ],
                                       double (&out)[NQ]) {
   if constexpr (ORDER == kMixed) {
-    float fh[NQ], fl[NQ];
+    using F = LineFit<NQ>;
+    constexpr int A = F::A, B = F::B, DEG = F::DEG;
+    const double f0 = f[0];
+    const double b = f[NQ - 1] - f0;
+    double rr[NQ];
+    double inv2 = 0.0, inv3 = 0.0;
+    {
+      double t[NQ];
+      sgpr_row<NQ>(Drow, NQ, f0, t);  // t_l; end slots: the Newton divisors
+      inv2 = t[0];
+      inv3 = t[NQ - 1];
 #pragma unroll
-    for (int l = 0; l < NQ; ++l) split_f64(f[l], fh[l], fl[l]);
+      for (int l = 1; l < NQ - 1; ++l) rr[l] = __builtin_fma(-b, t[l], f[l] - f0);
+    }
+    double c2 = 0.0, c3 = 0.0;
+    if constexpr (DEG >= 2) {
+      c2 = rr[A] * inv2;
+      double p2[NQ];
+      sgpr_row<NQ>(Drow, NQ + 1, c2, p2);
+#pragma unroll
+      for (int l = 1; l < NQ - 1; ++l)
+        if (l != A) rr[l] = __builtin_fma(-c2, p2[l], rr[l]);
+    }
+    if constexpr (DEG >= 3) {
+      c3 = rr[B] * inv3;
+      double p3[NQ];
+      sgpr_row<NQ>(Drow, NQ + 2, c3, p3);
+#pragma unroll
+      for (int l = 1; l < NQ - 1; ++l)
+        if (!F::fit(l)) rr[l] = __builtin_fma(-c3, p3[l], rr[l]);
+    }
+    // A line whose remainder is not small against the line itself (next to
+    // a discontinuity, or a field not resolved by the element) is
+    // contracted in FP64 instead: FP32 would round it at 2^-24 of the field.
+    double rmax = 0.0, fmx = 0.0;
+#pragma unroll
+    for (int l = 0; l < NQ; ++l) {
+      fmx = fmax(fmx, fabs(f[l]));
+      if (!F::fit(l)) rmax = fmax(rmax, fabs(rr[l]));
+    }
+    const bool rough = rmax > kMixedRough * fmx;
+    float r[NQ];
+#pragma unroll
+    for (int l = 0; l < NQ; ++l) r[l] = F::fit(l) ? 0.0f : (float)rr[l];
+    const double half_b = 0.5 * b;
 #pragma unroll
     for (int k = 0; k < NQ; ++k) {
       float dh[NQ], dl[NQ];
-      sgpr_row_split<NQ>(Drow, k, k >= kSweepDepth ? out[k - kSweepDepth] : f[0], dh, dl);
-      float acc = 0.0f, cor = 0.0f;
+      sgpr_row_split<NQ>(Drow, k, k >= kSweepDepth ? out[k - kSweepDepth] : f0, dh, dl);
+      double g = half_b;
+      if constexpr (DEG >= 2) g = __builtin_fma(c2, dword_pair(dh[0], dh[NQ - 1]), g);
+      if constexpr (DEG >= 3) g = __builtin_fma(c3, dword_pair(dl[0], dl[NQ - 1]), g);
+      if (__builtin_expect(rough, 0)) {
+        // D = D_hi + D_lo to 2^-48 relative
+        double acc = 0.0;
 #pragma unroll
-      for (int l = 0; l < NQ; ++l) {
-        acc = __builtin_fmaf(dh[l], fh[l], acc);
-        cor = __builtin_fmaf(dh[l], fl[l], __builtin_fmaf(dl[l], fh[l], cor));
+        for (int l = 1; l < NQ - 1; ++l) {
+          if (F::fit(l)) continue;
+          acc = __builtin_fma((double)dh[l] + (double)dl[l], rr[l], acc);
+        }
+        out[k] = acc + g;
+      } else {
+        float acc = 0.0f, cor = 0.0f;
+#pragma unroll
+        for (int l = 1; l < NQ - 1; ++l) {
+          if (F::fit(l)) continue;
+          acc = __builtin_fmaf(dh[l], r[l], acc);
+          cor = __builtin_fmaf(dl[l], r[l], cor);
+        }
+        out[k] = ((double)acc + (double)cor) + g;
       }
-      out[k] = (double)acc + (double)cor;
     }
     return;
   }
@@ -665,24 +749,8 @@ __device__ __forceinline__ void pencil_compute(PT &P, const double *__restrict__
     const double f3 = Bf[2 * FPAD + pf(j, k)];  // in(j,k,4)
     const double f1 = Bf[3 * FPAD + pf(j, k)];  // in(j,k,2)
     double L[6];
-    if constexpr (ORDER == kMixed) {
-      // lfac holds the split vectors as FP32: [l_-hi, l_+hi, l_-lo, l_+lo]
-      static_assert(FACT_LIFT, "the mixed form lifts with the 1D vectors");
-      const float *lf = reinterpret_cast<const float *>(lfac);
-      const int li[6] = {j, NQ + i, NQ + j, i, k, NQ + k};
-      const double fv[6] = {f0, f1, f2, f3, flux4, flux5};
-      float acc = 0.0f, cor = 0.0f;
-#pragma unroll
-      for (int f = 0; f < 6; ++f) {
-        float fh, fl;
-        split_f64(fv[f], fh, fl);
-        const float lh = lf[li[f]], ll = lf[2 * NQ + li[f]];
-        acc = __builtin_fmaf(lh, fh, acc);
-        cor = __builtin_fmaf(lh, fl, __builtin_fmaf(ll, fh, cor));
-      }
-      L[0] = (double)acc + (double)cor;  // the whole lift term
-      L[1] = L[2] = L[3] = L[4] = L[5] = 0.0;
-    } else if constexpr (FACT_LIFT) {
+    static_assert(ORDER != kMixed || FACT_LIFT, "the mixed form lifts with the 1D vectors");
+    if constexpr (FACT_LIFT) {
       // l_-(normal index) for -y,-x,-z; l_+ for +x,+y,+z; lfac = [l_-, l_+]
       L[0] = lfac[j];
       L[1] = lfac[NQ + i];
@@ -703,18 +771,19 @@ __device__ __forceinline__ void pencil_compute(PT &P, const double *__restrict__
         L[2 * m + 1] = pr.y;
       }
     }
-    // tensorprod_Lift_hexahedral order (mod_dg_optr_kernel.f90:277-283)
-    double lift;
-    if constexpr (ORDER == kMixed) {
-      lift = L[0];
-    } else {
-      lift = L[0] * f0;
-      lift = madd<FMA>(L[1], f1, lift);
-      lift = madd<FMA>(L[2], f2, lift);
-      lift = madd<FMA>(L[3], f3, lift);
-      lift = madd<FMA>(L[4], flux4, lift);
-      lift = madd<FMA>(L[5], flux5, lift);
-    }
+    // tensorprod_Lift_hexahedral order (mod_dg_optr_kernel.f90:277-283).
+    // The mixed form lifts in FP64 too (fused): a lift term has no
+    // cancellation to take out, so FP32 would round it at 2^-24 of the
+    // face-flux jump, which dominates the error next to a discontinuity,
+    // and six FP64 FMAs per node cost fewer instructions than a split FP32
+    // form.
+    constexpr bool kFuseLift = FMA || ORDER == kMixed;
+    double lift = L[0] * f0;
+    lift = madd<kFuseLift>(L[1], f1, lift);
+    lift = madd<kFuseLift>(L[2], f2, lift);
+    lift = madd<kFuseLift>(L[3], f3, lift);
+    lift = madd<kFuseLift>(L[4], flux4, lift);
+    lift = madd<kFuseLift>(L[5], flux5, lift);
     // dqdt = -(Es1 Dx + Es2 Dy + Es3 Dz + Lift) (mod_advect3d_eq.f90:194-198)
     double sum;
     if constexpr (LEAN) {

@@ -183,23 +183,26 @@ def test_ensemble_every_order(gpu, pmax, threads):
 
 
 def test_ensemble_mixed_precision(gpu, small):
-    """DGK_FLAG_MIXED in the ensemble (FP32 split contractions and lift):
-    every instance within the mixed bound of the oracle (1e-8 rel-L2 after 2
-    steps, and not FP64's bits), and the instances of order 1, 3, 7 equal to a
-    single mixed-precision solver, whose arithmetic is the same."""
+    """DGK_FLAG_MIXED in the ensemble: every instance within the mixed-
+    precision target of the oracle after 2 steps (<= 1e-9 rel-L2), and the
+    instances of order 1, 3, 7 equal to a single mixed-precision solver,
+    whose arithmetic is the same."""
     say("[ens] mixed precision")
     ens = _ensemble(small, dgk.FLAG_MIXED)
     ens.run(2)
+    errs = []
     for i, s in enumerate(small):
         got = ens.get_field(i)
         err = rel_l2(got, _oracle(s, 2))
-        assert 1e-14 < err <= 1e-8, (i, s["p"], err)
+        errs.append(err)
         if s["p"] in (1, 3, 7):
             sv = dgk.Solver.structured(*s["ne"], s["p"], s["D"], s["L"], flags=dgk.FLAG_MIXED)
             sv.set_fields(s["q"], *(np.full_like(s["q"], c) for c in s["vel"]))
             sv.run(s["dt"], 2)
             assert np.array_equal(got, sv.get_field()), i
             sv.close()
+    say("[ens] mixed rel-L2 after 2 steps: " + " ".join(f"{e:.2e}" for e in errs))
+    assert all(e <= 1e-9 for e in errs), errs
     ens.close()
     with pytest.raises(dgk.DGKError) as e:
         dgk.Ensemble(small, flags=dgk.FLAG_MIXED | dgk.FLAG_FMA)
@@ -250,23 +253,34 @@ def test_ensemble_errors(gpu):
 
 
 def test_ensemble_workload_properties(gpu):
-    """The 64-instance workload (150 M DOF), 5 steps: finite, mass conserved
-    per instance, and p=3 instances bit-exact against the oracle."""
+    """The 64-instance workload (150 M DOF) for its 50 steps: finite, mass
+    conserved per instance, p=3 instances bit-exact against the oracle; the
+    mixed-precision ensemble on the same workload within 1e-9 rel-L2 of the
+    FP64 ensemble for every instance after the 50 steps."""
     inst = ic.ensemble_workload(**{k: v for k, v in ic.ENSEMBLE.items() if k in ("n", "seed")})
-    say(f"[ens] workload, {len(inst)} instances")
+    nstep = ic.ENSEMBLE["nstep"]
+    say(f"[ens] workload, {len(inst)} instances, {nstep} steps")
     fields = []
     for s in inst:
         x1D, _, _ = dgk.read_operator(s["p"])
         m = dgk.mesh_setup(*s["ne"], s["p"], x1D=x1D, tables=False)
         fields.append((ic.blob(m["pos_en"], s["centre"], s["width"]), x1D))
-    ens = dgk.Ensemble(inst, flags=dgk.FLAG_CONST_VELOCITY)
-    for i, s in enumerate(inst):
-        q = fields[i][0]
-        ens.set_fields(i, q, *(np.full_like(q, c) for c in s["vel"]))
-    ens.set_dt([s["dt"] for s in inst])
-    ens.tune(2)
-    ens.run(5)
+
+    def build(flags):
+        ens = dgk.Ensemble(inst, flags=flags)
+        for i, s in enumerate(inst):
+            q = fields[i][0]
+            ens.set_fields(i, q, *(np.full_like(q, c) for c in s["vel"]))
+        ens.set_dt([s["dt"] for s in inst])
+        ens.tune(2)
+        ens.run(nstep)
+        return ens
+
+    ens = build(dgk.FLAG_CONST_VELOCITY)
+    say("[ens] mixed precision workload")
+    mix = build(dgk.FLAG_CONST_VELOCITY | dgk.FLAG_MIXED)
     checked = 0
+    worst = 0.0
     for i, s in enumerate(inst):
         got = ens.get_field(i)
         assert np.isfinite(got).all(), i
@@ -278,9 +292,14 @@ def test_ensemble_workload_properties(gpu):
         q0 = fields[i][0]
         scale = float(np.dot(w3, np.abs(q0).sum(axis=1)))
         assert abs(float(np.dot(w3, got.sum(axis=1) - q0.sum(axis=1)))) <= 1e-13 * scale, i
+        err = rel_l2(mix.get_field(i), got)
+        worst = max(worst, err)
+        assert 0.0 < err <= 1e-9, (i, s["p"], err)
         if s["p"] == 3 and checked < 2:
-            want = _oracle(dict(s, q=np.asfortranarray(q0)), 5)
+            want = _oracle(dict(s, q=np.asfortranarray(q0)), nstep)
             assert np.array_equal(got, want), i
             checked += 1
+    say(f"[ens] mixed vs FP64 worst instance rel-L2 {worst:.3e}")
     assert checked == 2
     ens.close()
+    mix.close()
