@@ -204,6 +204,19 @@ int dgk_autotune(dgk_solver *s, double dt, int nrep, int *variant_out,
 int dgk_diagnostics(dgk_solver *s, const double *w1D, const double *ref,
                     double *out);
 
+/* dgk_diagnostics that allocates nothing.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The same
+ * reductions in the same order (the same bits) as dgk_diagnostics, but the
+ * eight results go to out_dev, a device array of 8 doubles, in stream order
+ * on the solver's stream, and the call does not wait for them.  The
+ * per-block partial sums live in a file-scope device array of 1024 x 8
+ * doubles (64 KiB per GPU, held until the process ends; the reduction never
+ * uses more than 1024 blocks, whatever the field's size).  Every call on one
+ * GPU shares that array: calls that overlap on one GPU race. */
+int dgk_diagnostics_noalloc(dgk_solver *s, const double *w1D, const double *ref,
+                            double *out_dev);
+
 /* ---- stateless drop-in of the mesh-level tendency ----------------------- */
 /* Same arguments and semantics as advect3d_eq_cal_tend
  * (mod_advect3d_eq.f90:41-45); host arrays; q,u,v,w must carry their halos.

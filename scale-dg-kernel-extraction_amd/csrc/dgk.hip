@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 using namespace dgk;
@@ -200,13 +201,13 @@ __global__ void __launch_bounds__(256) diag_final_kernel(const double *part, int
 
 namespace dgk {
 
-int device_diagnostics(const double *q, const double *ref, long long n, int Nq,
+// The two reduction kernels of the diagnostics on stream st: per-block
+// partials into part (kDiag doubles per block, at most max_blocks blocks;
+// every block writes its own slot before the final kernel reads it, so part
+// needs no clearing), the eight results into res (device).
+int diagnostics_launch(const double *q, const double *ref, long long n, int Nq,
                        const double es[3], const double *escale, const double *w1D,
-                       double *scratch, void *stream, double *out) {
-  constexpr int kMaxBlocks = 1024;
-  static_assert(kDiag * kMaxBlocks + kDiag == kDiagScratch, "scratch layout");
-  hipStream_t st = (hipStream_t)stream;
-  double *part = scratch, *res = scratch + kDiag * kMaxBlocks;
+                       double *part, int max_blocks, hipStream_t st, double *res) {
   if (Nq < 2 || Nq > 16) return fail(DGK_ERR_ARG, "Nq %d outside 2..16", Nq);
   DiagArgs A{};
   A.q = q;
@@ -218,17 +219,66 @@ int device_diagnostics(const double *q, const double *ref, long long n, int Nq,
   A.NpNe = n;
   A.Nq = Nq;
   A.Np = Nq * Nq * Nq;
-  const int nb = (int)std::min<long long>(kMaxBlocks, (n + 255) / 256);
+  const int nb = (int)std::min<long long>(max_blocks, (n + 255) / 256);
+  if (nb < 1 || nb > max_blocks)
+    return fail(DGK_ERR_ARG, "diagnostics: %d partial blocks for %lld values outside 1..%d", nb,
+                n, max_blocks);
   auto kern = ref ? (escale ? diag_partial_kernel<true, true> : diag_partial_kernel<true, false>)
                  : (escale ? diag_partial_kernel<false, true> : diag_partial_kernel<false, false>);
   hipLaunchKernelGGL(kern, dim3(nb), dim3(256), sizeof(double) * A.Np, st, A, part);
   HIPOK(hipGetLastError());
   hipLaunchKernelGGL(diag_final_kernel, dim3(1), dim3(256), 0, st, part, nb, res);
   HIPOK(hipGetLastError());
+  return DGK_OK;
+}
+
+int device_diagnostics(const double *q, const double *ref, long long n, int Nq,
+                       const double es[3], const double *escale, const double *w1D,
+                       double *scratch, void *stream, double *out) {
+  constexpr int kMaxBlocks = 1024;
+  static_assert(kDiag * kMaxBlocks + kDiag == kDiagScratch, "scratch layout");
+  hipStream_t st = (hipStream_t)stream;
+  double *part = scratch, *res = scratch + kDiag * kMaxBlocks;
+  int rc = diagnostics_launch(q, ref, n, Nq, es, escale, w1D, part, kMaxBlocks, st, res);
+  if (rc) return rc;
   HIPOK(hipMemcpyAsync(out, res, kDiag * sizeof(double), hipMemcpyDeviceToHost, st));
   HIPOK(hipStreamSynchronize(st));
   return DGK_OK;
 }
+
+// Scratch of dgk_diagnostics_noalloc: the per-block partials of the reduction
+// (kDiag x kNoAllocBlocks doubles, 64 KiB).  Each GPU the library runs on
+// holds its own copy until the process ends.  In host code `scratch` only
+// names the symbol: the device address of a GPU's copy comes from
+// hipGetSymbolAddress with that GPU current (noalloc_scratch).
+constexpr int kNoAllocBlocks = 1024;
+static __device__ __align__(256) double scratch[kDiag * kNoAllocBlocks];
+
+constexpr int kMaxDevices = 64;
+
+// Device address of `scratch` on GPU dev, looked up once per device (with
+// dev current) and kept.
+int noalloc_scratch(int dev, double **ptr) {
+  static std::mutex mu;
+  static double *addr[kMaxDevices] = {};
+  if (dev < 0 || dev >= kMaxDevices)
+    return fail(DGK_ERR_ARG, "device %d outside 0..%d", dev, kMaxDevices - 1);
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[dev]) {
+    int cur = -1;
+    HIPOK(hipGetDevice(&cur));
+    if (cur != dev) HIPOK(hipSetDevice(dev));
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != dev) HIPOK(hipSetDevice(cur));
+    if (e != hipSuccess || !p)
+      return fail(DGK_ERR_HIP, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+    addr[dev] = (double *)p;
+  }
+  *ptr = addr[dev];
+  return DGK_OK;
+}
+
 }  // namespace dgk
 
 namespace {
@@ -426,6 +476,7 @@ struct dgk_solver {
   int cur = 0;  // buffer of q
   int c0 = 2;   // buffer of q0 (never cur)
   hipStream_t stream = nullptr;
+  int device = 0;  // the GPU current at creation (the stream's)
   StageKernel sk;
   // one captured step per (cur, c0) it starts from, and the (cur, c0) it ends in
   hipGraphExec_t graph[9] = {};
@@ -835,7 +886,8 @@ int dgk_create(dgk_solver **out, int Nq, int Ne, int NeA, int NfpTot, const doub
   // the mixed form lifts with the 1D vectors
   s->flags = (flags & DGK_FLAG_MIXED) ? (flags | DGK_FLAG_FACTORED_LIFT) : flags;
   int rc = DGK_OK;
-  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+  if (hipGetDevice(&s->device) != hipSuccess ||
+      hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
     delete s;
     return fail(DGK_ERR_HIP, "hipStreamCreate failed");
   }
@@ -893,7 +945,8 @@ int dgk_create_structured(dgk_solver **out, int NeX, int NeY, int NeZ, int p, do
   s->NeX = NeX;
   s->NeY = NeY;
   s->NeZ = NeZ;
-  if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
+  if (hipGetDevice(&s->device) != hipSuccess ||
+      hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
     delete s;
     return fail(DGK_ERR_HIP, "hipStreamCreate failed");
   }
@@ -1123,6 +1176,19 @@ int dgk_diagnostics(dgk_solver *s, const double *w1D, const double *ref, double 
   return device_diagnostics(s->q[s->cur], ref, (long long)s->Np * s->Ne, s->Nq,
                             s->nbr ? s->es : ones, s->nbr ? nullptr : s->escale, w1D, s->diag,
                             s->stream, out);
+}
+
+int dgk_diagnostics_noalloc(dgk_solver *s, const double *w1D, const double *ref,
+                            double *out_dev) {
+  if (!s || !w1D || !out_dev) return fail(DGK_ERR_ARG, "bad argument");
+  if (!s->fields_set) return fail(DGK_ERR_STATE, "dgk_set_fields has not been called");
+  double *part = nullptr;
+  int rc = noalloc_scratch(s->device, &part);
+  if (rc) return rc;
+  const double ones[3] = {1.0, 1.0, 1.0};
+  return diagnostics_launch(s->q[s->cur], ref, (long long)s->Np * s->Ne, s->Nq,
+                            s->nbr ? s->es : ones, s->nbr ? nullptr : s->escale, w1D, part,
+                            kNoAllocBlocks, s->stream, out_dev);
 }
 
 int dgk_autotune(dgk_solver *s, double dt, int nrep, int *variant_out, double *ms_out) {

@@ -38,6 +38,7 @@ EXPORTS = (
     "dgk_ensemble_set_fields", "dgk_ensemble_get_field", "dgk_ensemble_set_dt",
     "dgk_ensemble_step", "dgk_ensemble_run", "dgk_ensemble_time_stages",
     "dgk_ensemble_tune", "dgk_ensemble_query", "dgk_autotune", "dgk_diagnostics",
+    "dgk_diagnostics_noalloc",
     "dgk_ensemble_diagnostics",
     "dgk_matvec_dir_batched", "dgk_gradlike_dirXYZ_batched", "dgk_divlike_dirXYZ_var5_batched",
     "dgk_mesh_setup_device",
@@ -95,6 +96,7 @@ def lib():
         "dgk_gradlike_dirXYZ_batched": ([i, i] + [vp] * 7, i),
         "dgk_divlike_dirXYZ_var5_batched": ([i, i] + [vp] * 9, i),
         "dgk_diagnostics": ([vp, vp, vp, vp], i),
+        "dgk_diagnostics_noalloc": ([vp, vp, vp, vp], i),
         "dgk_ensemble_create": ([vp, i] + [vp] * 5 + [C.c_char_p, i, i], i),
         "dgk_ensemble_destroy": ([vp], i),
         "dgk_ensemble_set_fields": ([vp, i] + [vp] * 4, i),
