@@ -85,24 +85,57 @@ def mass(q, w3, J):
     return float(J * np.dot(w3, q.sum(axis=1))), float(J * np.dot(w3, np.abs(q).sum(axis=1)))
 
 
-def cpu_baseline(threads, nstep, spec, q):
-    """Times the reference's step loop (compiled Fortran, oracle/_ref) for
-    nstep steps of the workload itself (its mesh, order, velocity and dt),
-    starting from the owned field q(Np,Ne); falls back to the C restatement
-    (oracle/liboracle.so) where the reference binary is absent.  Only the
-    step loop is timed (the reference's own timer), not its mesh setup or
-    the file exchange."""
+def host_cpu():
+    """The host's CPU model, logical CPUs, sockets, NUMA nodes and the CPUs
+    this process may run on."""
+    model, sockets = None, set()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name") and model is None:
+                    model = line.split(":", 1)[1].strip()
+                elif line.startswith("physical id"):
+                    sockets.add(line.split(":", 1)[1].strip())
+    except OSError:
+        pass
+    try:
+        numa = len([d for d in os.listdir("/sys/devices/system/node") if d.startswith("node")
+                    and d[4:].isdigit()])
+    except OSError:
+        numa = None
+    return {"cpu_model": model, "logical_cpus": os.cpu_count(), "sockets": len(sockets) or None,
+            "numa_nodes": numa, "cpus_allowed": len(os.sched_getaffinity(0))}
+
+
+def cpu_baseline(threads, nstep, nrep, spec, q):
+    """Times the reference's step loop (its own Fortran, compiled from source
+    into oracle/_ref) with both of its operator types, GENERAL and OPT1:
+    nrep runs of nstep steps of the workload itself (its mesh, order,
+    velocity and dt) from the owned field q(Np,Ne), on `threads` OpenMP
+    threads spread over cores (OMP_PROC_BIND=spread, OMP_PLACES=cores); the
+    median per type.  Falls back to the C restatement (oracle/liboracle.so,
+    OPT1 order) where the reference program is absent.  Only the step loop
+    is timed (the reference's own loop), not its mesh setup or file
+    exchange."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     import oracle as O
     ne, p, vel = spec["ne"], spec["p"], spec["vel"]
     u, v, w = (np.full_like(q, c) for c in vel)
     dt = ic.cfl_dt(p, vel, ne)
     dof_updates = q.size * 3 * nstep
-    sample = (f"{ne[0]}x{ne[1]}x{ne[2]} elements, p={p}, {nstep} steps of the workload "
-              f"({dof_updates / 1e6:.1f} M DOF-updates)")
+    sample = (f"{ne[0]}x{ne[1]}x{ne[2]} elements, p={p}: {nrep} runs of {nstep} steps of the "
+              f"workload per operator type ({dof_updates / 1e6:.1f} M DOF-updates a run), "
+              f"median")
+    env = {"OMP_PROC_BIND": "spread", "OMP_PLACES": "cores"}
+    out = {"unit": "GDOF-updates/s", "cores": threads, "sample": sample, **host_cpu(),
+           "omp": dict(env, OMP_NUM_THREADS=str(threads))}
     if O.ref_available():
-        _, _, sec, _ = O.ref_run(*ne, p, q, u, v, w, dt, nstep, "OPT1", threads=threads)
-        kind = "reference"
+        secs, used = O.ref_bench(*ne, p, q, u, v, w, dt, nstep, nrep, threads=threads, env=env)
+        out["cores"] = used
+        for t in ("GENERAL", "OPT1"):
+            med = float(np.median(secs[t]))
+            out[t] = {"value": dof_updates / med / 1e9, "seconds": secs[t]}
+        out["kind"] = "reference"
     else:
         os.environ["OMP_NUM_THREADS"] = str(threads)
         om = O.Mesh(*ne, p)
@@ -112,10 +145,12 @@ def cpu_baseline(threads, nstep, spec, q):
             f[:, :om.Ne] = a
             om.update_halo(f)
             F.append(f)
-        sec = om.run(*F, dt, nstep, O.OPT1)
-        kind = "port"
-    return {"value": dof_updates / sec / 1e9, "unit": "GDOF-updates/s", "cores": threads,
-            "kind": kind, "sample": sample, "seconds": sec}
+        secs = [om.run(*(x.copy(order="F") for x in F), dt, nstep, O.OPT1) for _ in range(nrep)]
+        out["OPT1"] = {"value": dof_updates / float(np.median(secs)) / 1e9, "seconds": secs}
+        out["kind"] = "port"
+    # the headline baseline is the reference's default type, OPT1
+    out["value"] = out["OPT1"]["value"]
+    return out
 
 
 def traffic_from_profiles(workload, variant):
@@ -177,13 +212,13 @@ def mixed_workload(workload, x1D, D, L, spec, nstage=30):
                         ("uniform_velocity", VARIANTS["mixed"] | dgk.FLAG_CONST_VELOCITY)):
         s = dgk.Solver.structured(*ne, p, D, L, flags=flags)
         s.set_fields(q, *uvw)
-        s.run(dt, 1)
+        var, _ = s.autotune(dt, nrep=6)
         ms = s.time_stages(dt, nstage)
         s.close()
         bpd = BYTES_PER_DOF_CV if flags & dgk.FLAG_CONST_VELOCITY else BYTES_PER_DOF
         dof = ne[0] * ne[1] * ne[2] * (p + 1) ** 3
         res[name] = {"gdof_updates_per_s": round(dof / (ms * 1e-3) / 1e9, 2),
-                     "kernel_us": round(ms * 1e3, 1),
+                     "kernel_us": round(ms * 1e3, 1), "variant": var,
                      "hbm_frac": round(bpd * dof / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
     a = dgk.Solver.structured(*ne, p, D, L, flags=VARIANTS["mixed"])
     b = dgk.Solver.structured(*ne, p, D, L, flags=VARIANTS["exact"])
@@ -512,7 +547,8 @@ def main():
     ap.add_argument("--variant", default="factored", choices=sorted(VARIANTS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--copy-calibration", action="store_true")
     ap.add_argument("--no-error", action="store_true",
                     help="skip the nstep error run against the bit-exact variant")
@@ -643,10 +679,14 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        # the CPUs this job may use: the OpenMP share the host gives it (16
+        # per GPU on the GPU boxes, where OMP_NUM_THREADS is set), else every
+        # CPU it is allowed to run on
+        threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS") or 0) or \
+            len(os.sched_getaffinity(0))
         log(f"cpu baseline on {threads} threads")
-        cpu = cpu_baseline(threads, args.cpu_steps, spec, qf)
-        log(f"cpu baseline {cpu['value']:.4f} GDOF-updates/s ({cpu['kind']})")
+        cpu = cpu_baseline(threads, args.cpu_steps, args.cpu_reps, spec, qf)
+        log(f"cpu baseline OPT1 {cpu['value']:.4f} GDOF-updates/s ({cpu['kind']})")
 
     out = {
         "metric": METRIC,

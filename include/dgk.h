@@ -34,6 +34,7 @@ extern "C" {
 #define DGK_ERR_HIP 5        /* HIP runtime error                         */
 #define DGK_ERR_NOMEM 6      /* device allocation failed                  */
 #define DGK_ERR_STATE 7      /* call not valid in the solver's state      */
+#define DGK_ERR_NONFINITE 8  /* the state holds a NaN or an infinity      */
 
 /* Operator evaluation types.  GENERAL and OPT1 are the reference's
  * (mod_dg_optr_kernel.f90:39-42); HIP is the device path added here.  The
@@ -168,7 +169,10 @@ int dgk_rk_stage(dgk_solver *s, int stage, double dt);
 /* One full time step (three stages) replayed from a captured hipGraph. */
 int dgk_step(dgk_solver *s, double dt);
 /* nstep time steps; if minmax is non-NULL it receives (min,max) of the owned
- * q after every step, 2*nstep doubles (the main.f90:76-79 diagnostics). */
+ * q after every step, 2*nstep doubles (the main.f90:76-79 diagnostics), and
+ * a state with a NaN or an infinity makes the call return DGK_ERR_NONFINITE
+ * (naming the first such step) after all nstep steps and minmax are done; a
+ * NaN shows as min and max of that step.  Without minmax nothing is checked. */
 int dgk_run(dgk_solver *s, double dt, int nstep, double *minmax);
 int dgk_synchronize(dgk_solver *s);
 /* what: 0 = connectivity mode (1 compressed, 0 tables), 1 = Ne, 2 = Np,
@@ -183,11 +187,13 @@ int dgk_query(dgk_solver *s, int what, int64_t *value);
 int dgk_num_variants(dgk_solver *s, int *n);
 int dgk_set_variant(dgk_solver *s, int variant);
 /* Times `nstage` consecutive stage launches with HIP events on the solver's
- * stream; returns the mean milliseconds per stage launch. */
+ * stream; returns the mean milliseconds per stage launch.  The launches are
+ * whole SSP-RK3 steps (stage 1, 2, 3, 1, ...): nstage must be a multiple of
+ * 3, and the state advances by nstage/3 steps. */
 int dgk_time_stages(dgk_solver *s, double dt, int nstage, double *ms_per_stage);
 /* Autotuner: times every tuning variant (workgroup shape, element order,
- * store flavour; results are the same bits) over nrep stages at this dt,
- * keeps the fastest and leaves the state as it was.  variant_out and
+ * store flavour; results are the same bits) over nrep stages (rounded up to
+ * whole steps) at this dt, keeps the fastest and leaves the state as it was.  variant_out and
  * ms_out (average stage time of the winner) may be NULL. */
 int dgk_autotune(dgk_solver *s, double dt, int nrep, int *variant_out,
                  double *ms_out);
@@ -298,8 +304,8 @@ int dgk_ensemble_set_dt(dgk_ensemble *s, const double *dt);
 int dgk_ensemble_step(dgk_ensemble *s);
 /* nstep steps, then waits for the device */
 int dgk_ensemble_run(dgk_ensemble *s, int nstep);
-/* average device time of one stage launch over nstage launches (the state
- * advances by those stages) */
+/* average device time of one stage launch over nstage launches: whole
+ * SSP-RK3 steps (nstage a multiple of 3), which advance the state */
 int dgk_ensemble_time_stages(dgk_ensemble *s, int nstage, double *ms_per_stage);
 /* Measures each order class alone (nrep stage launches; the state is left
  * unchanged) and splits the persistent grid between the classes so that

@@ -209,10 +209,11 @@ def _unlimited_stack():
     resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
 
 
-def _ref(args, cwd, threads=None):
+def _ref(args, cwd, threads=None, extra_env=None):
     env = dict(os.environ)
     if threads:
         env["OMP_NUM_THREADS"] = str(threads)
+    env.update(extra_env or {})
     r = subprocess.run([REF_HARNESS] + [str(a) for a in args], cwd=cwd, env=env,
                        capture_output=True, text=True, preexec_fn=_unlimited_stack)
     if r.returncode:
@@ -304,6 +305,21 @@ def write_conf(path, NeX, NeY, NeZ, p, dt, nstep, opt="OPT1", output_interval=No
         f.write(f'  DGOptrKernel_OptType = "{opt}"\n')
         f.write(f"  dt = {dt:.17e}".replace("e", "D") + ",\n")
         f.write(f"  nstep = {nstep},\n  output_interval = {oi},\n/\n")
+
+
+def ref_bench(NeX, NeY, NeZ, p, q, u, v, w, dt, nstep, nrep, threads=None, env=None):
+    """Times the reference step loop (nstep steps) nrep times from the same
+    owned fields q,u,v,w (Np,Ne), with GENERAL then with OPT1, in one run of
+    the harness.  Returns ({"GENERAL": [s...], "OPT1": [s...]}, threads)."""
+    with _workdir() as d:
+        conf, icf, outf = (os.path.join(d, n) for n in ("in.conf", "ic.bin", "out.bin"))
+        write_conf(conf, NeX, NeY, NeZ, p, dt, nstep)
+        with open(icf, "wb") as f:
+            for a in (q, u, v, w):
+                f.write(np.asfortranarray(a, dtype=np.float64).tobytes(order="F"))
+        _ref(["bench", conf, icf, outf, nrep], d, threads, env)
+        raw = np.fromfile(outf, dtype=np.float64)
+    return {"GENERAL": raw[:nrep].tolist(), "OPT1": raw[nrep:2 * nrep].tolist()}, int(raw[-1])
 
 
 def ref_run(NeX, NeY, NeZ, p, q, u, v, w, dt, nstep, opt="OPT1", mode="run",

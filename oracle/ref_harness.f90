@@ -59,8 +59,14 @@ program ref_harness
     call get_command_argument(3, a2)
     call get_command_argument(4, a3)
     call do_run(trim(mode), trim(a1), trim(a2), trim(a3))
+  case ("bench")
+    call get_command_argument(2, a1)
+    call get_command_argument(3, a2)
+    call get_command_argument(4, a3)
+    call get_command_argument(5, a4); read(a4,*) nelem
+    call do_bench(trim(a1), trim(a2), trim(a3), nelem)
   case default
-    write(*,'(A)') "usage: ref_harness mesh|optr|tend|run ..."
+    write(*,'(A)') "usage: ref_harness mesh|optr|tend|run|bench ..."
     error stop 2
   end select
 
@@ -297,5 +303,74 @@ contains
     write(fid) t1 - t0
     close(fid)
   end subroutine do_run
+
+  !> bench: the main.f90 step loop timed nrep times from the same initial
+  !! state with each operator type, GENERAL then OPT1 (one mesh setup and one
+  !! read of the initial fields for all of them); writes the 2*nrep loop
+  !! times (GENERAL's, then OPT1's) and the thread count.
+  subroutine do_bench(conf, icfile, outfile, nrep)
+    character(len=*), intent(in) :: conf, icfile, outfile
+    integer, intent(in) :: nrep
+    integer :: NeX = 4, NeY = 4, NeZ = 4, PolyOrder = 3
+    integer :: nstep = 1, output_interval = 1
+    real(RP) :: dt = 1.0e-3_RP
+    real(RP) :: vel_x = 1.0_RP, vel_y = 1.0_RP, vel_z = 1.0_RP
+    character(len=8) :: DGOptrKernel_OptType = "OPT1"
+    namelist /PARAM_ADVECT3D/ NeX, NeY, NeZ, PolyOrder, dt, nstep, &
+         output_interval, vel_x, vel_y, vel_z, DGOptrKernel_OptType
+    real(RP), allocatable :: q(:,:), qi(:,:), q0(:,:), dqdt(:,:), u(:,:), v(:,:), w(:,:)
+    real(RP), allocatable :: secs(:)
+    character(len=8), parameter :: types(2) = [character(len=8) :: "GENERAL", "OPT1"]
+    integer :: fid, istep, stage, ke, it, r
+    real(RP) :: t0
+
+    open(newunit=fid, file=conf, status='old', action='read')
+    read(fid, nml=PARAM_ADVECT3D)
+    close(fid)
+    call mesh_setup(NeX, NeY, NeZ, PolyOrder, 1.0_RP, 1.0_RP, 1.0_RP)
+    allocate(q(Np,NeA), qi(Np,Ne), q0(Np,NeA), dqdt(Np,NeA))
+    allocate(u(Np,NeA), v(Np,NeA), w(Np,NeA), secs(2*nrep))
+    q = 0.0_RP; u = 0.0_RP; v = 0.0_RP; w = 0.0_RP; q0 = 0.0_RP; dqdt = 0.0_RP
+    open(newunit=fid, file=icfile, access='stream', form='unformatted', status='old')
+    read(fid) qi(:,1:Ne), u(:,1:Ne), v(:,1:Ne), w(:,1:Ne)
+    close(fid)
+    call update_halo(u)
+    call update_halo(v)
+    call update_halo(w)
+    do it = 1, 2
+      call dg_optr_kernel_setup(types(it))
+      do r = 1, nrep
+        !$omp parallel do
+        do ke = 1, Ne
+          q(:,ke) = qi(:,ke)
+        end do
+        call update_halo(q)
+        t0 = omp_get_wtime()
+        do istep = 1, nstep
+          !$omp parallel do
+          do ke = 1, Ne
+            q0(:,ke) = q(:,ke)
+          end do
+          do stage = 1, RK_nstage
+            call update_halo(q)
+            call advect3d_eq_cal_tend(dqdt, q, u, v, w, D1D, D1D_tr, Lift_mat, &
+                 VMapM, VMapP, normal_fn, Escale, Fscale, Nq, Np, NfpTot, Ne, NeA)
+            !$omp parallel do
+            do ke = 1, Ne
+              q(:,ke) = rk_a(stage) * q0(:,ke) &
+                   + rk_b(stage) * ( q(:,ke) + dt * dqdt(:,ke) )
+            end do
+          end do
+        end do
+        secs((it-1)*nrep + r) = omp_get_wtime() - t0
+        write(*,'(A,A8,A,I3,A,ES16.6)') "BENCH ", types(it), " rep", r, " seconds", &
+             secs((it-1)*nrep + r)
+      end do
+    end do
+    open(newunit=fid, file=outfile, access='stream', form='unformatted', status='replace')
+    write(fid) secs
+    write(fid) real(omp_get_max_threads(), RP)
+    close(fid)
+  end subroutine do_bench
 
 end program ref_harness

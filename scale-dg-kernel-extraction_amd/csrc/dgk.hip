@@ -30,26 +30,37 @@ __global__ void halo_gather_kernel(double *f, const int32_t *src, int n, long lo
   if (ib < n) f[base + ib] = f[src[ib] - 1];
 }
 
+// min/max that keep a NaN operand (fmin/fmax drop it): a state that has gone
+// NaN shows in the per-step line instead of vanishing from it.  On numbers
+// they are fmin/fmax.
+__device__ __forceinline__ double nan_min(double a, double b) {
+  return (a != a || b != b) ? a + b : fmin(a, b);
+}
+__device__ __forceinline__ double nan_max(double a, double b) {
+  return (a != a || b != b) ? a + b : fmax(a, b);
+}
+
 // Two-pass (min,max) of the owned state: per-block partials, then one block.
 __global__ void __launch_bounds__(256)
     minmax_partial_kernel(const double *q, long long n, double *part) {
   __shared__ double smn[256], smx[256];
   double mn = __builtin_inf(), mx = -__builtin_inf();
-  // unrolled so that several loads of a thread are in flight at once
+  // unrolled so that several loads of a thread are in flight at once; a NaN
+  // value sticks (no comparison with a NaN holds)
 #pragma unroll 4
   for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
        t += (long long)gridDim.x * blockDim.x) {
     const double x = q[t];
-    mn = x < mn ? x : mn;
-    mx = x > mx ? x : mx;
+    mn = (x < mn || x != x) ? x : mn;
+    mx = (x > mx || x != x) ? x : mx;
   }
   smn[threadIdx.x] = mn;
   smx[threadIdx.x] = mx;
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
     if (threadIdx.x < s) {
-      smn[threadIdx.x] = fmin(smn[threadIdx.x], smn[threadIdx.x + s]);
-      smx[threadIdx.x] = fmax(smx[threadIdx.x], smx[threadIdx.x + s]);
+      smn[threadIdx.x] = nan_min(smn[threadIdx.x], smn[threadIdx.x + s]);
+      smx[threadIdx.x] = nan_max(smx[threadIdx.x], smx[threadIdx.x + s]);
     }
     __syncthreads();
   }
@@ -64,16 +75,16 @@ __global__ void __launch_bounds__(256)
   __shared__ double smn[256], smx[256];
   double mn = __builtin_inf(), mx = -__builtin_inf();
   for (int t = threadIdx.x; t < nb; t += blockDim.x) {
-    mn = fmin(mn, part[2 * t]);
-    mx = fmax(mx, part[2 * t + 1]);
+    mn = nan_min(mn, part[2 * t]);
+    mx = nan_max(mx, part[2 * t + 1]);
   }
   smn[threadIdx.x] = mn;
   smx[threadIdx.x] = mx;
   __syncthreads();
   for (int s = 128; s > 0; s >>= 1) {
     if (threadIdx.x < s) {
-      smn[threadIdx.x] = fmin(smn[threadIdx.x], smn[threadIdx.x + s]);
-      smx[threadIdx.x] = fmax(smx[threadIdx.x], smx[threadIdx.x + s]);
+      smn[threadIdx.x] = nan_min(smn[threadIdx.x], smn[threadIdx.x + s]);
+      smx[threadIdx.x] = nan_max(smx[threadIdx.x], smx[threadIdx.x + s]);
     }
     __syncthreads();
   }
@@ -102,11 +113,11 @@ struct DiagArgs {
 };
 
 __device__ __forceinline__ void diag_combine(double *a, const double *b) {
-  a[0] = fmin(a[0], b[0]);
-  a[1] = fmax(a[1], b[1]);
+  a[0] = nan_min(a[0], b[0]);
+  a[1] = nan_max(a[1], b[1]);
 #pragma unroll
   for (int t = 2; t < 7; ++t) a[t] += b[t];
-  a[7] = fmax(a[7], b[7]);
+  a[7] = nan_max(a[7], b[7]);
 }
 
 __device__ __forceinline__ void diag_block_reduce(double (&v)[kDiag], double *out) {
@@ -168,8 +179,8 @@ __global__ void __launch_bounds__(256) diag_partial_kernel(const DiagArgs A, dou
     const double wj = J * wn[nd];
     nd += dn;
     if (nd >= A.Np) nd -= A.Np;
-    v[0] = fmin(v[0], x);
-    v[1] = fmax(v[1], x);
+    v[0] = nan_min(v[0], x);
+    v[1] = nan_max(v[1], x);
     v[2] += wj * x;
     v[3] += wj * fabs(x);
     v[4] += x * x;
@@ -177,7 +188,7 @@ __global__ void __launch_bounds__(256) diag_partial_kernel(const DiagArgs A, dou
       const double r = A.r[t], d = x - r;
       v[5] += d * d;
       v[6] += r * r;
-      v[7] = fmax(v[7], fabs(d));
+      v[7] = nan_max(v[7], fabs(d));
     }
   }
   diag_block_reduce(v, part + kDiag * blockIdx.x);
@@ -1119,6 +1130,11 @@ int dgk_run(dgk_solver *s, double dt, int nstep, double *minmax) {
     HIPOK(hipMemcpyAsync(minmax, s->mm, 2 * (size_t)nstep * sizeof(double),
                          hipMemcpyDeviceToHost, s->stream));
   HIPOK(hipStreamSynchronize(s->stream));
+  if (minmax)
+    for (int it = 0; it < nstep; ++it)
+      if (!std::isfinite(minmax[2 * it]) || !std::isfinite(minmax[2 * it + 1]))
+        return fail(DGK_ERR_NONFINITE, "the state is not finite after step %d (min %g, max %g)",
+                    it + 1, minmax[2 * it], minmax[2 * it + 1]);
   return DGK_OK;
 }
 
@@ -1160,9 +1176,19 @@ int dgk_set_variant(dgk_solver *s, int variant) {
   }
   if (variant < -1 || variant >= pencil_variants(s))
     return fail(DGK_ERR_ARG, "variant %d outside -1..%d", variant, pencil_variants(s) - 1);
+  // configure_launch fills s->sk (and the tile table) for s->variant: on
+  // failure the previous variant and its launch descriptor are put back
+  const int old_variant = s->variant;
+  const StageKernel old_sk = s->sk;
   s->variant = variant;
+  const int rc = configure_launch(s);
+  if (rc || !s->sk.launch || (tiled_order(s) && !s->tile_order)) {
+    s->variant = old_variant;
+    s->sk = old_sk;
+    return rc ? rc : fail(DGK_ERR_STATE, "variant %d has no stage kernel", variant);
+  }
   drop_graphs(s);
-  return configure_launch(s);
+  return DGK_OK;
 }
 
 int dgk_diagnostics(dgk_solver *s, const double *w1D, const double *ref, double *out) {
@@ -1195,18 +1221,14 @@ int dgk_autotune(dgk_solver *s, double dt, int nrep, int *variant_out, double *m
   if (!s || nrep < 1) return fail(DGK_ERR_ARG, "bad argument");
   if (!s->fields_set) return fail(DGK_ERR_STATE, "dgk_set_fields has not been called");
   const int nvar = pencil_variants(s);
-  if (nvar <= 1) {
-    if (variant_out) *variant_out = 0;
-    if (ms_out) return dgk_time_stages(s, dt, nrep, ms_out);
-    return DGK_OK;
-  }
+  const int nstage = (nrep + 2) / 3 * 3;  // whole steps
   // the trial stages advance the state: keep q and q0 aside and restore them
   HIPOK(hipStreamSynchronize(s->stream));
   const size_t bytes = (size_t)s->Np * (size_t)s->NeA * sizeof(double);
   double *save = nullptr;
   int rc = dalloc(&save, 2 * (size_t)s->Np * (size_t)s->NeA);
   if (rc) return rc;
-  const int cur0 = s->cur, c00 = s->c0;
+  const int cur0 = s->cur, c00 = s->c0, var0 = s->variant;
   int best = -1;
   double best_ms = 0.0;
   if (hipMemcpy(save, s->q[cur0], bytes, hipMemcpyDeviceToDevice) != hipSuccess ||
@@ -1215,9 +1237,9 @@ int dgk_autotune(dgk_solver *s, double dt, int nrep, int *variant_out, double *m
     rc = fail(DGK_ERR_HIP, "autotune: state copy failed");
   for (int v = 0; v < nvar && !rc; ++v) {
     double ms = 0.0;
-    if ((rc = dgk_set_variant(s, v))) break;
+    if (nvar > 1 && (rc = dgk_set_variant(s, v))) break;
     if ((rc = dgk_time_stages(s, dt, 3, &ms))) break;  // warm-up
-    if ((rc = dgk_time_stages(s, dt, nrep, &ms))) break;
+    if ((rc = dgk_time_stages(s, dt, nstage, &ms))) break;
     if (best < 0 || ms < best_ms) {
       best = v;
       best_ms = ms;
@@ -1225,36 +1247,45 @@ int dgk_autotune(dgk_solver *s, double dt, int nrep, int *variant_out, double *m
   }
   s->cur = cur0;
   s->c0 = c00;
-  if (!rc && (hipMemcpy(s->q[cur0], save, bytes, hipMemcpyDeviceToDevice) != hipSuccess ||
-              hipMemcpy(s->q[c00], save + bytes / sizeof(double), bytes, hipMemcpyDeviceToDevice) !=
-                  hipSuccess))
-    rc = fail(DGK_ERR_HIP, "autotune: state restore failed");
+  if (hipMemcpy(s->q[cur0], save, bytes, hipMemcpyDeviceToDevice) != hipSuccess ||
+      hipMemcpy(s->q[c00], save + bytes / sizeof(double), bytes, hipMemcpyDeviceToDevice) !=
+          hipSuccess) {
+    if (!rc) rc = fail(DGK_ERR_HIP, "autotune: state restore failed");
+  }
   (void)hipFree(save);
-  if (rc) return rc;
-  if ((rc = dgk_set_variant(s, best))) return rc;
-  if (variant_out) *variant_out = best;
+  if (rc) {
+    if (nvar > 1) (void)dgk_set_variant(s, var0);
+    return rc;
+  }
+  if (nvar > 1 && (rc = dgk_set_variant(s, best))) return rc;
+  if (variant_out) *variant_out = nvar > 1 ? best : 0;
   if (ms_out) *ms_out = best_ms;
   return DGK_OK;
 }
 
 int dgk_time_stages(dgk_solver *s, double dt, int nstage, double *ms_per_stage) {
   if (!s || !ms_per_stage || nstage < 1) return fail(DGK_ERR_ARG, "bad argument");
+  if (nstage % 3)
+    return fail(DGK_ERR_ARG, "nstage %d is not a whole number of SSP-RK3 steps (3 stages each)",
+                nstage);
   if (!s->fields_set) return fail(DGK_ERR_STATE, "dgk_set_fields has not been called");
-  hipEvent_t e0, e1;
-  HIPOK(hipEventCreate(&e0));
-  HIPOK(hipEventCreate(&e1));
-  HIPOK(hipEventRecord(e0, s->stream));
-  for (int it = 0; it < nstage; ++it) {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = DGK_OK;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess ||
+      hipEventRecord(e0, s->stream) != hipSuccess)
+    rc = fail(DGK_ERR_HIP, "time_stages: event setup failed");
+  for (int it = 0; it < nstage && !rc; ++it) {
     const int st = it % 3;
-    int rc = launch_stage(s, st == 0 ? kModeStage1 : kModeStageN, kRkA[st], kRkB[st], dt);
-    if (rc) return rc;
+    rc = launch_stage(s, st == 0 ? kModeStage1 : kModeStageN, kRkA[st], kRkB[st], dt);
   }
-  HIPOK(hipEventRecord(e1, s->stream));
-  HIPOK(hipEventSynchronize(e1));
   float ms = 0.f;
-  HIPOK(hipEventElapsedTime(&ms, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  if (!rc && (hipEventRecord(e1, s->stream) != hipSuccess ||
+              hipEventSynchronize(e1) != hipSuccess ||
+              hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+    rc = fail(DGK_ERR_HIP, "time_stages: event timing failed");
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (rc) return rc;
   *ms_per_stage = ms / nstage;
   return DGK_OK;
 }

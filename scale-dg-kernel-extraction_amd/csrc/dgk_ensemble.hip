@@ -660,21 +660,24 @@ int dgk_ensemble_run(dgk_ensemble *s, int nstep) {
 
 int dgk_ensemble_time_stages(dgk_ensemble *s, int nstage, double *ms_per_stage) {
   if (!s || nstage < 1 || !ms_per_stage) return fail(DGK_ERR_ARG, "bad argument");
+  if (nstage % 3)
+    return fail(DGK_ERR_ARG, "nstage %d is not a whole number of SSP-RK3 steps (3 stages each)",
+                nstage);
   if (!s->fields_set) return fail(DGK_ERR_STATE, "no fields set");
-  hipEvent_t e0, e1;
-  HIPOK(hipEventCreate(&e0));
-  HIPOK(hipEventCreate(&e1));
-  HIPOK(hipEventRecord(e0, s->stream));
-  for (int it = 0; it < nstage; ++it) {
-    int rc = ens_launch(s, it % 3);
-    if (rc) return rc;
-  }
-  HIPOK(hipEventRecord(e1, s->stream));
-  HIPOK(hipEventSynchronize(e1));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = DGK_OK;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess ||
+      hipEventRecord(e0, s->stream) != hipSuccess)
+    rc = fail(DGK_ERR_HIP, "ensemble time_stages: event setup failed");
+  for (int it = 0; it < nstage && !rc; ++it) rc = ens_launch(s, it % 3);
   float ms = 0.f;
-  HIPOK(hipEventElapsedTime(&ms, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  if (!rc && (hipEventRecord(e1, s->stream) != hipSuccess ||
+              hipEventSynchronize(e1) != hipSuccess ||
+              hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+    rc = fail(DGK_ERR_HIP, "ensemble time_stages: event timing failed");
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (rc) return rc;
   *ms_per_stage = ms / nstage;
   return DGK_OK;
 }
@@ -683,10 +686,10 @@ int dgk_ensemble_tune(dgk_ensemble *s, int nrep, double *ms_per_class) {
   if (!s || nrep < 1) return fail(DGK_ERR_ARG, "bad argument");
   if (!s->fields_set) return fail(DGK_ERR_STATE, "no fields set");
   HIPOK(hipStreamSynchronize(s->stream));
-  hipEvent_t e0, e1;
-  HIPOK(hipEventCreate(&e0));
-  HIPOK(hipEventCreate(&e1));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
   int rc = DGK_OK;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+    rc = fail(DGK_ERR_HIP, "ensemble tune: hipEventCreate failed");
   const int units = std::max(s->slots / kNumXcd, 1);
   for (int c = 0; c < s->ncls && !rc; ++c) {
     // stage 1 of class c alone on the whole grid: reads q and writes the
@@ -714,8 +717,8 @@ int dgk_ensemble_tune(dgk_ensemble *s, int nrep, double *ms_per_class) {
     // class takes alone times the workgroups it had then
     s->cls_weight[c] = t * u;
   }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
   if (rc) return rc;
   ens_drop_graphs(s);
   ens_partition(s);

@@ -21,7 +21,7 @@ LIB_PATH = os.path.join(PKG, "lib", "libdgk.so")
 OPERATOR_DIR = os.path.join(REPO, "operator_data")
 
 OK = 0
-ERR_ARG, ERR_OPTTYPE, ERR_ORDER, ERR_IO, ERR_HIP, ERR_NOMEM, ERR_STATE = range(1, 8)
+ERR_ARG, ERR_OPTTYPE, ERR_ORDER, ERR_IO, ERR_HIP, ERR_NOMEM, ERR_STATE, ERR_NONFINITE = range(1, 9)
 TYPE_GENERAL, TYPE_OPT1, TYPE_HIP = 1, 2, 3
 FLAG_FMA, FLAG_FORCE_TABLES, FLAG_NODE_KERNEL, FLAG_FACTORED_LIFT = 0x1, 0x2, 0x4, 0x8
 FLAG_CONST_VELOCITY = 0x10

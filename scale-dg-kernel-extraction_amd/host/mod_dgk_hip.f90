@@ -14,18 +14,28 @@ module mod_dgk_hip
   private
 
   integer, public, parameter :: DGK_OK = 0
+  integer, public, parameter :: DGK_ERR_ARG = 1, DGK_ERR_OPTTYPE = 2, DGK_ERR_ORDER = 3, &
+       DGK_ERR_IO = 4, DGK_ERR_HIP = 5, DGK_ERR_NOMEM = 6, DGK_ERR_STATE = 7, &
+       DGK_ERR_NONFINITE = 8
   integer, public, parameter :: DGK_TYPE_GENERAL = 1, DGK_TYPE_OPT1 = 2, DGK_TYPE_HIP = 3
   integer, public, parameter :: DGK_FLAG_FMA = 1, DGK_FLAG_FORCE_TABLES = 2, &
-       DGK_FLAG_NODE_KERNEL = 4, DGK_FLAG_FACTORED_LIFT = 8, DGK_FLAG_CONST_VELOCITY = 16
-  integer, public, parameter :: DGK_FIELD_Q = 0, DGK_FIELD_Q0 = 1, DGK_FIELD_DQDT = 2
+       DGK_FLAG_NODE_KERNEL = 4, DGK_FLAG_FACTORED_LIFT = 8, DGK_FLAG_CONST_VELOCITY = 16, &
+       DGK_FLAG_MIXED = 32
+  integer, public, parameter :: DGK_FIELD_Q = 0, DGK_FIELD_Q0 = 1, DGK_FIELD_DQDT = 2, &
+       DGK_FIELD_U = 3, DGK_FIELD_V = 4, DGK_FIELD_W = 5
 
-  public :: dgk_optr_kernel_setup, dgk_read_operator, dgk_mesh_sizes, dgk_mesh_setup
+  public :: dgk_version, dgk_optr_kernel_setup, dgk_read_operator, dgk_mesh_sizes
+  public :: dgk_mesh_setup, dgk_mesh_setup_device
   public :: dgk_create, dgk_create_structured, dgk_destroy
-  public :: dgk_set_fields, dgk_get_field, dgk_cal_tend, dgk_rk_stage, dgk_step, dgk_run
-  public :: dgk_synchronize, dgk_advect3d_eq_cal_tend, dgk_error_message
-  public :: dgk_autotune, dgk_diagnostics
+  public :: dgk_set_fields, dgk_get_field, dgk_field_ptr, dgk_cal_tend, dgk_rk_stage, dgk_step
+  public :: dgk_run, dgk_synchronize, dgk_query, dgk_num_variants, dgk_set_variant
+  public :: dgk_time_stages, dgk_advect3d_eq_cal_tend, dgk_error_message
+  public :: dgk_autotune, dgk_diagnostics, dgk_diagnostics_noalloc
+  public :: dgk_divlike_dirXYZ_batched, dgk_lift_hexahedral_batched, dgk_matvec_dir_batched
+  public :: dgk_gradlike_dirXYZ_batched, dgk_divlike_dirXYZ_var5_batched
   public :: dgk_ensemble_create, dgk_ensemble_destroy, dgk_ensemble_set_fields
-  public :: dgk_ensemble_get_field, dgk_ensemble_set_dt, dgk_ensemble_run, dgk_ensemble_tune
+  public :: dgk_ensemble_get_field, dgk_ensemble_set_dt, dgk_ensemble_step, dgk_ensemble_run
+  public :: dgk_ensemble_time_stages, dgk_ensemble_tune, dgk_ensemble_query
   public :: dgk_ensemble_diagnostics
 
   interface
@@ -218,6 +228,123 @@ module mod_dgk_hip
 
     type(c_ptr) function dgk_last_error() bind(C, name="dgk_last_error")
       import :: c_ptr
+    end function
+
+    integer(c_int) function dgk_version() bind(C, name="dgk_version")
+      import :: c_int
+    end function
+
+    ! the mesh tables built on the device (device arrays; any may be c_null_ptr)
+    integer(c_int) function dgk_mesh_setup_device(NeX, NeY, NeZ, p, domx, domy, domz, x1D, &
+         VMapM, VMapP, normal_fn, Escale, Fscale, pos_en, halo_src_map, stream) &
+         bind(C, name="dgk_mesh_setup_device")
+      import :: c_int, c_double, c_ptr
+      integer(c_int), value :: NeX, NeY, NeZ, p
+      real(c_double), value :: domx, domy, domz
+      real(c_double), intent(in) :: x1D(*)
+      type(c_ptr), value :: VMapM, VMapP, normal_fn, Escale, Fscale, pos_en, halo_src_map, stream
+    end function
+
+    ! device address of a field (Np*NeA doubles); q and q0 move with every stage
+    integer(c_int) function dgk_field_ptr(solver, which, dptr) bind(C, name="dgk_field_ptr")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: solver
+      integer(c_int), value :: which
+      type(c_ptr), intent(out) :: dptr
+    end function
+
+    integer(c_int) function dgk_query(solver, what, val) bind(C, name="dgk_query")
+      import :: c_int, c_int64_t, c_ptr
+      type(c_ptr), value :: solver
+      integer(c_int), value :: what
+      integer(c_int64_t), intent(out) :: val
+    end function
+
+    integer(c_int) function dgk_num_variants(solver, n) bind(C, name="dgk_num_variants")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: solver
+      integer(c_int), intent(out) :: n
+    end function
+
+    integer(c_int) function dgk_set_variant(solver, variant) bind(C, name="dgk_set_variant")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: solver
+      integer(c_int), value :: variant
+    end function
+
+    ! nstage a multiple of 3 (whole steps; the state advances)
+    integer(c_int) function dgk_time_stages(solver, dt, nstage, ms) bind(C, name="dgk_time_stages")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: solver
+      real(c_double), value :: dt
+      integer(c_int), value :: nstage
+      real(c_double), intent(out) :: ms
+    end function
+
+    ! dgk_diagnostics into the device array out_dev(8), without allocating or waiting
+    integer(c_int) function dgk_diagnostics_noalloc(solver, w1D, ref, out_dev) &
+         bind(C, name="dgk_diagnostics_noalloc")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: solver
+      real(c_double), intent(in) :: w1D(*)
+      type(c_ptr), value :: ref, out_dev
+    end function
+
+    ! the reference's operator kernels over nelem element blocks (device arrays)
+    integer(c_int) function dgk_divlike_dirXYZ_batched(Nq, nelem, type_id, Mat, Mat_tr, &
+         in_x, in_y, in_z, out_x, out_y, out_z, stream) bind(C, name="dgk_divlike_dirXYZ_batched")
+      import :: c_int, c_ptr
+      integer(c_int), value :: Nq, nelem, type_id
+      type(c_ptr), value :: Mat, Mat_tr, in_x, in_y, in_z, out_x, out_y, out_z, stream
+    end function
+
+    integer(c_int) function dgk_lift_hexahedral_batched(Nq, nelem, Lift, fin, fout, stream) &
+         bind(C, name="dgk_lift_hexahedral_batched")
+      import :: c_int, c_ptr
+      integer(c_int), value :: Nq, nelem
+      type(c_ptr), value :: Lift, fin, fout, stream
+    end function
+
+    integer(c_int) function dgk_matvec_dir_batched(Nq, nelem, dir, Mat, fin, fout, stream) &
+         bind(C, name="dgk_matvec_dir_batched")
+      import :: c_int, c_ptr
+      integer(c_int), value :: Nq, nelem, dir
+      type(c_ptr), value :: Mat, fin, fout, stream
+    end function
+
+    integer(c_int) function dgk_gradlike_dirXYZ_batched(Nq, nelem, Mat, Mat_tr, fin, &
+         out_x, out_y, out_z, stream) bind(C, name="dgk_gradlike_dirXYZ_batched")
+      import :: c_int, c_ptr
+      integer(c_int), value :: Nq, nelem
+      type(c_ptr), value :: Mat, Mat_tr, fin, out_x, out_y, out_z, stream
+    end function
+
+    integer(c_int) function dgk_divlike_dirXYZ_var5_batched(Nq, nelem, Mat, Mat_tr, &
+         in_x, in_y, in_z, out_x, out_y, out_z, stream) &
+         bind(C, name="dgk_divlike_dirXYZ_var5_batched")
+      import :: c_int, c_ptr
+      integer(c_int), value :: Nq, nelem
+      type(c_ptr), value :: Mat, Mat_tr, in_x, in_y, in_z, out_x, out_y, out_z, stream
+    end function
+
+    integer(c_int) function dgk_ensemble_step(ens) bind(C, name="dgk_ensemble_step")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ens
+    end function
+
+    integer(c_int) function dgk_ensemble_time_stages(ens, nstage, ms) &
+         bind(C, name="dgk_ensemble_time_stages")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ens
+      integer(c_int), value :: nstage
+      real(c_double), intent(out) :: ms
+    end function
+
+    integer(c_int) function dgk_ensemble_query(ens, what, val) bind(C, name="dgk_ensemble_query")
+      import :: c_int, c_int64_t, c_ptr
+      type(c_ptr), value :: ens
+      integer(c_int), value :: what
+      integer(c_int64_t), intent(out) :: val
     end function
   end interface
 

@@ -131,3 +131,49 @@ def test_device_mesh_build_equals_host(gpu, ne, p, dom):
         got = dev[k].cpu().numpy()
         want = np.asarray(m[k]).ravel(order="F")
         assert got.dtype == want.dtype and np.array_equal(got, want), k
+
+
+def test_nonfinite_state_is_reported(gpu):
+    """One NaN seeded in q: dgk_run(minmax) still runs its steps and fills
+    the per-step line, which shows the NaN (the reductions keep it), and
+    returns DGK_ERR_NONFINITE naming the first step; the diagnostics show the
+    NaN as min and max.  Without minmax nothing is checked (status 0)."""
+    import ctypes as C
+    s, x1D, _, _, q, vel = _solver((3, 2, 2), 3)
+    bad = q.copy(order="F")
+    bad[5, 7] = np.nan
+    dt = 1e-3
+    for with_mm in (True, False):
+        say(f"[nonfinite] minmax={with_mm}")
+        s.set_fields(bad, *(np.full_like(bad, c) for c in vel))
+        mm = np.zeros((2, 3), order="F")
+        rc = dgk.lib().dgk_run(s._h, dt, 3, mm.ctypes.data_as(C.c_void_p) if with_mm else None)
+        if with_mm:
+            assert rc == dgk.ERR_NONFINITE
+            assert b"after step 1" in dgk.lib().dgk_last_error()
+            assert np.isnan(mm).all()
+        else:
+            assert rc == 0
+        d = s.diagnostics(x1D)
+        assert np.isnan(d["min"]) and np.isnan(d["max"])
+    # a finite state after the failure: status 0 again
+    s.set_fields(q, *(np.full_like(q, c) for c in vel))
+    mm = s.run(dt, 3, minmax=True)
+    assert np.isfinite(mm).all()
+    s.close()
+
+
+def test_time_stages_whole_steps_and_autotune_state(gpu):
+    """dgk_time_stages times whole SSP-RK3 steps only (a count that is not a
+    multiple of 3 is refused), and the autotuner leaves the state as it was
+    also for a stage kernel with a single variant (thread-per-node)."""
+    s, x1D, D, L, q, vel = _solver((3, 3, 2), 5, flags=dgk.FLAG_NODE_KERNEL)
+    assert s.num_variants == 1
+    with pytest.raises(dgk.DGKError) as e:
+        s.time_stages(1e-3, 4)
+    assert e.value.code == dgk.ERR_ARG
+    before = s.get_field()
+    var, ms = s.autotune(1e-3, nrep=4)
+    assert var == 0 and ms > 0.0
+    assert np.array_equal(s.get_field(), before)
+    s.close()
