@@ -72,6 +72,17 @@ extern "C" {
                                       (compressed connectivity), no FMA
                                       flag */
 
+#define DGK_FLAG_MFMA 0x40          /* directional contractions on the matrix
+                                      cores: FP64 v_mfma_f64_16x16x4_f64, or
+                                      with DGK_FLAG_MIXED FP32
+                                      v_mfma_f32_16x16x4_f32 with operator
+                                      and operand split hi + lo (three
+                                      products); face fluxes, lift, residual
+                                      and RK in FP64.  PolyOrder 15 with
+                                      DGK_FLAG_CONST_VELOCITY on a compressed
+                                      mesh; implies the factored lift; fused
+                                      (not bit-identical) */
+
 /* Fields addressable through dgk_get_field / dgk_field_ptr. */
 #define DGK_FIELD_Q 0
 #define DGK_FIELD_Q0 1

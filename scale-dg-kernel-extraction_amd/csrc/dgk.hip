@@ -551,6 +551,7 @@ void tile_shape(const dgk_solver *s, int *tx, int *ty, long long *tseq) {
 // order x workgroup shape (Nq 13-16 have one shape).
 int pencil_variants(const dgk_solver *s) {
   if (!s->sk.pencil) return 1;
+  if (s->flags & DGK_FLAG_MFMA) return kPencilVariants;  // four or eight wavefronts
   return (s->Nq >= 13 || s->order == kMixed) ? kPencilWide : kPencilVariants;
 }
 
@@ -567,12 +568,18 @@ int configure_launch(dgk_solver *s) {
   if ((s->flags & DGK_FLAG_CONST_VELOCITY) && !use_pencil)
     return fail(DGK_ERR_ARG, "DGK_FLAG_CONST_VELOCITY needs the pencil kernel (compressed "
                              "connectivity, no DGK_FLAG_NODE_KERNEL)");
-  if (use_pencil)
+  if (s->flags & DGK_FLAG_MFMA) {
+    if (!use_pencil || s->Nq != 16 || !(s->flags & DGK_FLAG_CONST_VELOCITY))
+      return fail(DGK_ERR_ARG, "DGK_FLAG_MFMA needs PolyOrder 15, DGK_FLAG_CONST_VELOCITY and "
+                               "compressed connectivity (no DGK_FLAG_NODE_KERNEL)");
+    s->sk = pick_mfma(s->Nq, s->order == kMixed, (var & kPencilWide) != 0);
+  } else if (use_pencil) {
     s->sk = pick_pencil(s->Nq, s->order, s->fma, (s->flags & DGK_FLAG_FACTORED_LIFT) != 0,
                         (var & kPencilNtStore) != 0, (s->flags & DGK_FLAG_CONST_VELOCITY) != 0,
                         (var & kPencilWide) != 0);
-  else
+  } else {
     s->sk = pick_stage(s->Nq, s->order, s->fma, s->structured);
+  }
   if (!s->sk.kernel && s->order == kMixed)
     return fail(DGK_ERR_ARG, "DGK_FLAG_MIXED needs the pencil kernel (compressed "
                              "connectivity, no DGK_FLAG_NODE_KERNEL) without DGK_FLAG_FMA");
@@ -895,7 +902,7 @@ int dgk_create(dgk_solver **out, int Nq, int Ne, int NeA, int NfpTot, const doub
   s->order = (flags & DGK_FLAG_MIXED) ? kMixed : order;
   s->fma = (flags & DGK_FLAG_FMA) != 0;
   // the mixed form lifts with the 1D vectors
-  s->flags = (flags & DGK_FLAG_MIXED) ? (flags | DGK_FLAG_FACTORED_LIFT) : flags;
+  s->flags = (flags & (DGK_FLAG_MIXED | DGK_FLAG_MFMA)) ? (flags | DGK_FLAG_FACTORED_LIFT) : flags;
   int rc = DGK_OK;
   if (hipGetDevice(&s->device) != hipSuccess ||
       hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess) {
@@ -951,7 +958,7 @@ int dgk_create_structured(dgk_solver **out, int NeX, int NeY, int NeZ, int p, do
   s->order = (flags & DGK_FLAG_MIXED) ? kMixed : order;
   s->fma = (flags & DGK_FLAG_FMA) != 0;
   // the mixed form lifts with the 1D vectors
-  s->flags = (flags & DGK_FLAG_MIXED) ? (flags | DGK_FLAG_FACTORED_LIFT) : flags;
+  s->flags = (flags & (DGK_FLAG_MIXED | DGK_FLAG_MFMA)) ? (flags | DGK_FLAG_FACTORED_LIFT) : flags;
   s->structured = true;
   s->NeX = NeX;
   s->NeY = NeY;

@@ -40,5 +40,10 @@ StageKernel pick_pencil_group(int Nq, int order, bool fma, bool fact, bool cv, i
 StageKernel pick_pencil_group_low(int Nq, int order, bool fma, bool fact, bool cv, int epb);
 StageKernel pick_pencil_group_mid(int Nq, int order, bool fma, bool fact, bool cv, int epb);
 StageKernel pick_pencil_group_high(int Nq, int order, bool fma, bool fact, bool cv, int epb);
+// The matrix-core stage (dgk_mfma.hip) for Nq = 16 with uniform velocity:
+// FP64 MFMA contractions, or with `mixed` FP32 MFMA with hi/lo-split
+// operator and operand; `wide`: eight wavefronts per element instead of
+// four.  An empty descriptor for other Nq.
+StageKernel pick_mfma(int Nq, bool mixed, bool wide);
 
 }  // namespace dgk

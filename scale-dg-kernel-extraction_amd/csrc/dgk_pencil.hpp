@@ -486,6 +486,10 @@ constexpr int kPencilLinearOrder = 2;
 constexpr int kPencilWide = 4;
 constexpr int kPencilVariants = 8;
 
+// Lean stage: request the x-lines of u, the y-lines of v and q0 one phase
+// ahead of their use (see pencil_compute).
+constexpr bool kLeanEarlyLoads = true;
+
 template <bool NT>
 __device__ __forceinline__ void st(double *p, double v) {
   if constexpr (NT) __builtin_nontemporal_store(v, p);
@@ -622,6 +626,24 @@ __device__ __forceinline__ void pencil_compute(PT &P, const double *__restrict__
     flux5 = upwind_flux(qz[NQ - 1], qP5, w[NQ - 1], wP5, P.fs[5]);
   };
   if constexpr (!LEAN) z_phase(q);
+  // LEAN: every line a phase reads from memory (the x-lines of u, the y-lines
+  // of v, q0) is requested before the barrier that ends the previous phase,
+  // so that its latency overlaps that wait instead of heading the phase.
+  constexpr bool kEarly = LEAN && kLeanEarlyLoads;
+  double uxe[NQ], vye[NQ];
+  auto load_ux = [&] {
+#pragma unroll
+    for (int l = 0; l < NQ; ++l) uxe[l] = CV ? P.cvel[0] : P.u[ebase + l + NQ * pl];
+  };
+  auto load_vy = [&] {
+#pragma unroll
+    for (int l = 0; l < NQ; ++l) vye[l] = CV ? P.cvel[1] : P.v[ebase + la + NQ * l + NFP * lb];
+  };
+  auto load_q0 = [&] {
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) q0[k] = need_q0 ? ld_stream(P.q0 + base + NFP * k) : 0.0;
+  };
+  if constexpr (kEarly) load_ux();
   if constexpr (!QG) {
     if (act) {
 #pragma unroll
@@ -642,9 +664,9 @@ __device__ __forceinline__ void pencil_compute(PT &P, const double *__restrict__
     // ---- x phase: line (j,k) = (la,lb); S = Es1 D_x ------------------------
     // The x-line of u is read from memory (Nq consecutive values per lane).
     {
-      double qx[NQ], ux[NQ], fx[NQ], dx[NQ];
-#pragma unroll
-      for (int l = 0; l < NQ; ++l) ux[l] = CV ? P.cvel[0] : P.u[ebase + l + NQ * pl];
+      double qx[NQ], fx[NQ], dx[NQ];
+      if constexpr (!kEarly) load_ux();
+      const double (&ux)[NQ] = uxe;
 #pragma unroll
       for (int l = 0; l < NQ; ++l) {
         qx[l] = QG ? P.qin[ebase + l + NQ * pl] : Bq[px(l)];
@@ -660,13 +682,14 @@ __device__ __forceinline__ void pencil_compute(PT &P, const double *__restrict__
         for (int i = 0; i < NQ; ++i) BS[px(i)] = e1s * dx[i];
       }
     }
+    if constexpr (kEarly) load_vy();
     __syncthreads();
     // ---- y phase: line (i,k) = (la,lb); S += Es2 D_y -----------------------
     // The y-line of v is read from memory.
     {
-      double qy[NQ], vy[NQ], fy[NQ], dy[NQ];
-#pragma unroll
-      for (int l = 0; l < NQ; ++l) vy[l] = CV ? P.cvel[1] : P.v[ebase + la + NQ * l + NFP * lb];
+      double qy[NQ], fy[NQ], dy[NQ];
+      if constexpr (!kEarly) load_vy();
+      const double (&vy)[NQ] = vye;
 #pragma unroll
       for (int l = 0; l < NQ; ++l) {
         qy[l] = QG ? P.qin[ebase + la + NQ * l + NFP * lb] : Bq[py(l)];
@@ -685,6 +708,7 @@ __device__ __forceinline__ void pencil_compute(PT &P, const double *__restrict__
         }
       }
     }
+    if constexpr (kEarly) load_q0();
     __syncthreads();
   } else {
     // ---- y phase: line (i,k) = (la,lb) -------------------------------------
@@ -735,8 +759,7 @@ __device__ __forceinline__ void pencil_compute(PT &P, const double *__restrict__
   if (!valid) return;
   const int i = la, j = lb;
   if constexpr (LEAN) {
-#pragma unroll
-    for (int k = 0; k < NQ; ++k) q0[k] = need_q0 ? ld_stream(P.q0 + base + NFP * k) : 0.0;
+    if constexpr (!kEarly) load_q0();
 #pragma unroll
     for (int k = 0; k < NQ; ++k) q[k] = QG ? P.qin[base + NFP * k] : Bq[pz(k)];
     z_phase(q);

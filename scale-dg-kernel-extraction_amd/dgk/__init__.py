@@ -26,6 +26,7 @@ TYPE_GENERAL, TYPE_OPT1, TYPE_HIP = 1, 2, 3
 FLAG_FMA, FLAG_FORCE_TABLES, FLAG_NODE_KERNEL, FLAG_FACTORED_LIFT = 0x1, 0x2, 0x4, 0x8
 FLAG_CONST_VELOCITY = 0x10
 FLAG_MIXED = 0x20
+FLAG_MFMA = 0x40
 FIELD_Q, FIELD_Q0, FIELD_DQDT, FIELD_U, FIELD_V, FIELD_W = range(6)
 
 EXPORTS = (

@@ -20,7 +20,7 @@ module mod_dgk_hip
   integer, public, parameter :: DGK_TYPE_GENERAL = 1, DGK_TYPE_OPT1 = 2, DGK_TYPE_HIP = 3
   integer, public, parameter :: DGK_FLAG_FMA = 1, DGK_FLAG_FORCE_TABLES = 2, &
        DGK_FLAG_NODE_KERNEL = 4, DGK_FLAG_FACTORED_LIFT = 8, DGK_FLAG_CONST_VELOCITY = 16, &
-       DGK_FLAG_MIXED = 32
+       DGK_FLAG_MIXED = 32, DGK_FLAG_MFMA = 64
   integer, public, parameter :: DGK_FIELD_Q = 0, DGK_FIELD_Q0 = 1, DGK_FIELD_DQDT = 2, &
        DGK_FIELD_U = 3, DGK_FIELD_V = 4, DGK_FIELD_W = 5
 
