@@ -65,9 +65,15 @@ VARIANTS = {
     "exact_constvel": dgk.FLAG_CONST_VELOCITY,
     "fma_node": dgk.FLAG_FMA | dgk.FLAG_NODE_KERNEL,
     "fma_factored_constvel": dgk.FLAG_FMA | dgk.FLAG_FACTORED_LIFT | dgk.FLAG_CONST_VELOCITY,
-    # FP32 split contractions and lift (pencil kernels, every order), FP64 state
+    # FP32 contractions of each line's remainder after its cubic part
     "mixed": dgk.FLAG_MIXED,
+    # p=15, uniform velocity: contractions on the matrix cores, FP64 MFMA and
+    # FP32 MFMA with hi/lo-split operator and operand
+    "mfma_constvel": dgk.FLAG_MFMA | dgk.FLAG_CONST_VELOCITY,
+    "mfma_mixed_constvel": dgk.FLAG_MFMA | dgk.FLAG_MIXED | dgk.FLAG_CONST_VELOCITY,
 }
+# MI355X dense matrix peaks (TFLOP/s)
+MATRIX_PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
 
 
 def log(msg):
@@ -253,26 +259,60 @@ def mixed_workload(workload, x1D, D, L, spec, nstage=30):
     return res
 
 
-def single_workload(workload, variants, nstage=30):
+def flops_per_dof(p):
+    """Algorithmic flops per DOF-update: three sweeps (2 Nq each), 22 for
+    the flux products, combine, lift and RK, and the face fluxes (120/Nq)."""
+    nq = p + 1
+    return 6 * nq + 22 + 120 / nq
+
+
+def single_workload(workload, variants, nstage=30, errors=False):
     """Stage kernel time of a few variants on another workload (after
-    autotuning each), in GDOF-updates/s."""
+    autotuning each), in GDOF-updates/s and TFLOP/s; with `errors`, each
+    variant's rel-L2 against the bit-identical variant after the workload's
+    nstep (both run on the device from the same field)."""
     spec = ic.WORKLOADS[workload]
     ne, p, vel = spec["ne"], spec["p"], spec["vel"]
     x1D, D, L = dgk.read_operator(p)
     q = _fields(spec, x1D)
     dt = ic.cfl_dt(p, vel, ne)
-    res = {"NeX": ne[0], "PolyOrder": p}
+    uvw = [np.full_like(q, c) for c in vel]
+    res = {"NeX": ne[0], "PolyOrder": p, "flops_per_dof_update": flops_per_dof(p)}
+    ref = None
+    if errors:
+        ref = dgk.Solver.structured(*ne, p, D, L, flags=VARIANTS["exact"])
+        ref.set_fields(q, *uvw)
+        ref.run(dt, spec["nstep"])
     for name in variants:
         s = dgk.Solver.structured(*ne, p, D, L, flags=VARIANTS[name])
-        s.set_fields(q, *(np.full_like(q, c) for c in vel))
+        s.set_fields(q, *uvw)
         var, _ = s.autotune(dt, nrep=6)
         ms = s.time_stages(dt, nstage)
         bpd = BYTES_PER_DOF_CV if VARIANTS[name] & dgk.FLAG_CONST_VELOCITY else BYTES_PER_DOF
-        res[name] = {"gdof_updates_per_s": round(s.Ne * s.Np / (ms * 1e-3) / 1e9, 2),
-                     "kernel_us": round(ms * 1e3, 1), "variant": var,
-                     "hbm_frac": round(bpd * s.Ne * s.Np / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+        dofs = s.Ne * s.Np / (ms * 1e-3)
+        r = {"gdof_updates_per_s": round(dofs / 1e9, 2), "kernel_us": round(ms * 1e3, 1),
+             "variant": var, "hbm_frac": round(bpd * dofs / 1e9 / HBM_PEAK_GBS, 4),
+             "tflops": round(flops_per_dof(p) * dofs / 1e12, 2),
+             "fp64_peak_frac": round(flops_per_dof(p) * dofs / 1e12 / MATRIX_PEAK_TFLOPS["fp64"], 4)}
+        if VARIANTS[name] & dgk.FLAG_MFMA:
+            # flops the matrix cores execute: the three sweeps (2 Nq per DOF
+            # each), three products each in the hi/lo-split FP32 form
+            mixed = bool(VARIANTS[name] & dgk.FLAG_MIXED)
+            mf = 6 * (p + 1) * (3 if mixed else 1)
+            peak = MATRIX_PEAK_TFLOPS["fp32" if mixed else "fp64"]
+            r.update(matrix_tflops_executed=round(mf * dofs / 1e12, 2),
+                     matrix_peak_tflops=peak,
+                     matrix_peak_frac=round(mf * dofs / 1e12 / peak, 4))
+        if ref is not None:
+            s.set_fields(q, *uvw)
+            s.run(dt, spec["nstep"])
+            r["rel_l2_vs_exact_after_nstep"] = s.diagnostics(x1D, ref=ref)["rel_l2"]
+        res[name] = r
         s.close()
-        log(f"{workload} {name}: {res[name]['gdof_updates_per_s']} GDOF-updates/s")
+        log(f"{workload} {name}: {r['gdof_updates_per_s']} GDOF-updates/s")
+    if ref is not None:
+        ref.close()
+        res["nstep"] = spec["nstep"]
     return res
 
 
@@ -666,9 +706,11 @@ def main():
         # the FMA form with per-node velocity (each autotuned)
         extra["c3_64cube_p7_other_variants"] = single_workload(
             "c3_64cube_p7", ("factored_constvel", "fma_factored_constvel", "fma_factored"))
-        extra["c5_16cube_p15"] = single_workload("c5_16cube_p15", ("exact", "factored",
-                                                                   "factored_constvel",
-                                                                   "fma_factored_constvel"))
+        # the high-order configuration: FP64 on the vector units (exact,
+        # factored, FMA) against FP64 MFMA and FP32-split MFMA contractions
+        extra["c5_16cube_p15"] = single_workload(
+            "c5_16cube_p15", ("exact", "factored", "factored_constvel", "fma_factored_constvel",
+                              "mfma_constvel", "mfma_mixed_constvel", "mixed"), errors=True)
         # the two smaller configurations: the reference's CPU-sized case and
         # the 32^3 mesh of the headline order
         extra["c1_8cube_p3"] = single_workload("c1_8cube_p3", ("exact", "factored"))
