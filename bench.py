@@ -318,8 +318,8 @@ def single_workload(workload, variants, nstage=30, errors=False):
 
 def ensemble_workload(nstage=30):
     """The 64-instance ensemble (orders 3, 5, 7, 11, uneven meshes, uniform
-    per-instance velocity): one persistent launch per stage, grid split
-    tuned per order class; FP64 with fused multiply-adds, and the mixed
+    per-instance velocity): one persistent launch per stage, tile shape and
+    grid split tuned per order class (the untuned default shape beside it); FP64 with fused multiply-adds, and the mixed
     form with its error against the FP64 ensemble after the workload's 50
     steps."""
     inst = ic.ensemble_workload(**{k: v for k, v in ic.ENSEMBLE.items() if k in ("n", "seed")})
@@ -344,14 +344,22 @@ def ensemble_workload(nstage=30):
     ens.run(5)
     el = time.perf_counter() - t0
     dof = ens.query(0)
+    nt, epb = ens.shape
     res = {"instances": len(inst), "DOF": dof, "classes": ens.classes,
+           "tile_shape": {"threads": nt, "elems_per_item": epb},
            "gdof_updates_per_s_kernel": round(dof / (ms * 1e-3) / 1e9, 2),
            "gdof_updates_per_s_graph": round(dof * 15 / el / 1e9, 2),
            "variant": "FP64, fused multiply-adds, factored lift, uniform velocity per instance"}
+    ens.set_shape()  # the default shape, grid split by DOF
+    ms0 = ens.time_stages(nstage)
+    res["untuned"] = {"gdof_updates_per_s_kernel": round(dof / (ms0 * 1e-3) / 1e9, 2),
+                      "tile_shape": {"threads": ens.shape[0], "elems_per_item": ens.shape[1]},
+                      "split": "by DOF"}
     ens.close()
     ens = build(dgk.FLAG_CONST_VELOCITY | dgk.FLAG_MIXED)
     ms = ens.time_stages(nstage)
     res["mixed"] = {"gdof_updates_per_s_kernel": round(dof / (ms * 1e-3) / 1e9, 2),
+                    "tile_shape": {"threads": ens.shape[0], "elems_per_item": ens.shape[1]},
                     "variant": "FP32 contractions of the lines' remainders after their cubic "
                                "part, FP64 elsewhere, uniform velocity"}
     ens.close()

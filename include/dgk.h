@@ -318,14 +318,28 @@ int dgk_ensemble_run(dgk_ensemble *s, int nstep);
 /* average device time of one stage launch over nstage launches: whole
  * SSP-RK3 steps (nstage a multiple of 3), which advance the state */
 int dgk_ensemble_time_stages(dgk_ensemble *s, int nstage, double *ms_per_stage);
-/* Measures each order class alone (nrep stage launches; the state is left
- * unchanged) and splits the persistent grid between the classes so that
- * they finish together.  ms_per_class[number of classes] or NULL. */
+/* Chooses the tile shape and the grid split from measured times (nrep
+ * stage-1 launches per trial; the state is left unchanged): for each
+ * workgroup width the largest order allows (192 up to PolyOrder 12, 256),
+ * every order class tries its candidate elements per item (as many as fill
+ * the workgroup, about half, one) timed alone; the grid is split between the
+ * classes by those times, by DOF and by the geometric mean of the two, the
+ * whole stage is timed for each, the fastest width and split are kept, and
+ * single units of eight workgroups move between the largest class and the
+ * others while the whole stage gets faster.  Results are bit-identical for
+ * every shape and split.
+ * ms_per_class[number of classes]: each class's time alone in the chosen
+ * shape, or NULL. */
 int dgk_ensemble_tune(dgk_ensemble *s, int nrep, double *ms_per_class);
+/* Sets the tile shape by hand: threads per workgroup (192 or 256; 192 only
+ * without an instance of PolyOrder 13..15; 0: the default for the largest
+ * order) and elems_per_item[number of classes] (0 or NULL: as many as fill
+ * the workgroup).  Drops a previous tuning (the grid is split by DOF). */
+int dgk_ensemble_set_shape(dgk_ensemble *s, int threads, const int32_t *elems_per_item);
 /* what: 0 total DOF, 1 work items per stage, 2 workgroups, 3 LDS bytes,
  * 4 classes of equal order, 5 tuned (0/1), 6 threads per workgroup (192, or
- * 256 with an instance of PolyOrder 13..15), 16+c Nq of class c,
- * 32+c workgroups of class c */
+ * 256 with an instance of PolyOrder 13..15, unless tuned or set), 16+c Nq of
+ * class c, 32+c workgroups of class c, 48+c elements per item of class c */
 int dgk_ensemble_query(dgk_ensemble *s, int what, int64_t *value);
 /* dgk_diagnostics for instance i: out[8] over its owned state, against the
  * device field ref (Np,Ne) or NULL; w1D its LGL weights (Nq). */

@@ -9,18 +9,20 @@
 // runs items of one class only, so that it is compiled and scheduled as one
 // straight code path.  A workgroup of 192 threads (three wavefronts, each on
 // its own SIMD) runs an item as the pencil stage (one lane per node line) of
-// as many elements as fill it: 24, 12, 5, 3 or 1 for Nq = 2, 4, 6, 8, 12.
-// An ensemble with an instance of Nq 14..16 (196..256 lines per element)
-// uses workgroups of 256 threads instead.
+// up to as many elements as fill it: 48, 12, 5, 3 or 1 for Nq = 2, 4, 6, 8,
+// 12.  An ensemble with an instance of Nq 14..16 (196..256 lines per
+// element) uses workgroups of 256 threads instead; any ensemble may.
 // Within a class the items are split into eight contiguous ranges, one per
 // XCD, so that an instance's neighbouring elements stay in one L2.  The
-// split of the grid between the classes follows measured per-class stage
-// times (dgk_ensemble_tune) or, before tuning, the DOF count of each class.
+// split of the grid between the classes follows the DOF count of each
+// class, or, once dgk_ensemble_tune has run, the tile shape (workgroup width,
+// elements per item of each class) and split that measured fastest.
 #include "dgk_internal.hpp"
 #include "dgk_kernels.hpp"
 #include "dgk_pencil.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <map>
 #include <vector>
@@ -61,6 +63,7 @@ struct EnsArgs {
   const int2 *items;     // (instance, first element), grouped by class
   int ncls;
   int cls_nq[kEnsMaxClass];
+  int cls_epb[kEnsMaxClass];        // elements per item of each class
   int cls_block[kEnsMaxClass + 1];  // first workgroup of each class (multiples of 8)
   int cls_item[kEnsMaxClass + 1];   // first item of each class
   const StageParams *params;        // per instance, for this launch's stage and placement
@@ -70,33 +73,33 @@ bool ens_supported(int nq) { return nq >= 2 && nq <= 16; }
 // workgroup size for an ensemble whose largest order has max_nq nodes a line
 int ens_threads(int max_nq) { return max_nq <= 13 ? kEnsNarrow : kEnsWide; }
 
-// Elements per item (one workgroup): Nq^2 lanes per element, 24, 12, 5, 3
+// Most elements per item (one workgroup): Nq^2 lanes per element, 24, 12, 5, 3
 // and 1 elements for Nq 2, 4, 6, 8, 12.
 int ens_item_elems(int nq, int nt) { return nt / (nq * nq); }
 
 // LDS of an item (the ensemble stages q in LDS whatever the velocity)
 template <int NQ>
-size_t ens_lds_nq(int nt) {
-  return sizeof(double) * (nt / (NQ * NQ)) * PencilShape<NQ>::lean_elem(false);
+size_t ens_lds_nq() {
+  return sizeof(double) * PencilShape<NQ>::lean_elem(false);
 }
 
-size_t ens_item_lds(int nq, int nt) {
+size_t ens_elem_lds(int nq) {
   switch (nq) {
-  case 2: return ens_lds_nq<2>(nt);
-  case 3: return ens_lds_nq<3>(nt);
-  case 4: return ens_lds_nq<4>(nt);
-  case 5: return ens_lds_nq<5>(nt);
-  case 6: return ens_lds_nq<6>(nt);
-  case 7: return ens_lds_nq<7>(nt);
-  case 8: return ens_lds_nq<8>(nt);
-  case 9: return ens_lds_nq<9>(nt);
-  case 10: return ens_lds_nq<10>(nt);
-  case 11: return ens_lds_nq<11>(nt);
-  case 12: return ens_lds_nq<12>(nt);
-  case 13: return ens_lds_nq<13>(nt);
-  case 14: return ens_lds_nq<14>(nt);
-  case 15: return ens_lds_nq<15>(nt);
-  default: return ens_lds_nq<16>(nt);
+  case 2: return ens_lds_nq<2>();
+  case 3: return ens_lds_nq<3>();
+  case 4: return ens_lds_nq<4>();
+  case 5: return ens_lds_nq<5>();
+  case 6: return ens_lds_nq<6>();
+  case 7: return ens_lds_nq<7>();
+  case 8: return ens_lds_nq<8>();
+  case 9: return ens_lds_nq<9>();
+  case 10: return ens_lds_nq<10>();
+  case 11: return ens_lds_nq<11>();
+  case 12: return ens_lds_nq<12>();
+  case 13: return ens_lds_nq<13>();
+  case 14: return ens_lds_nq<14>();
+  case 15: return ens_lds_nq<15>();
+  default: return ens_lds_nq<16>();
   }
 }
 
@@ -111,11 +114,12 @@ typedef __attribute__((address_space(4))) const int CInt;
 
 // The items [first, first+count) of one class on nb workgroups (lb = this
 // workgroup's index among them; lb % 8 is its XCD).  An item is the lean
-// pencil stage of the elements first .. first+EPB-1 of an instance; lanes
-// beyond EPB*Nq^2 only join the barriers.
+// pencil stage of the elements first .. first+epb-1 of an instance (epb at
+// most EPB, the elements whose lines fill the workgroup); lanes beyond
+// epb*Nq^2 only join the barriers.
 template <int NT, int NQ, int ORDER, bool FMA, bool FACT, bool CV>
-__device__ __forceinline__ void ens_items(const EnsArgs &A, int first, int count, int lb, int nb,
-                                          double *smem) {
+__device__ __forceinline__ void ens_items(const EnsArgs &A, int first, int count, int epb, int lb,
+                                          int nb, double *smem) {
   constexpr int NFP = NQ * NQ, EPB = NT / NFP;
   static_assert(EPB >= 1, "an element's lines must fit the workgroup");
   constexpr int PER = PencilShape<NQ>::lean_elem(false);  // q staged in LDS
@@ -131,7 +135,7 @@ __device__ __forceinline__ void ens_items(const EnsArgs &A, int first, int count
     // in registers through every item
     int tid = threadIdx.x;
     asm volatile("" : "+v"(tid));
-    const bool act = tid < EPB * NFP;
+    const bool act = tid < epb * NFP;
     const int el = act ? tid / NFP : 0;
     const int pl = act ? tid - el * NFP : 0;
     long long e = first_el + el;
@@ -156,26 +160,26 @@ __global__ void __launch_bounds__(NT, NT == kEnsNarrow ? 3 : 2) ensemble_kernel(
   const int b0 = A.cls_block[c], nb = A.cls_block[c + 1] - b0;
   if ((int)blockIdx.x >= A.cls_block[A.ncls] || nb <= 0) return;  // uniform
   const int lb = blockIdx.x - b0;
-  const int first = A.cls_item[c], count = A.cls_item[c + 1] - first;
+  const int first = A.cls_item[c], count = A.cls_item[c + 1] - first, epb = A.cls_epb[c];
   switch (A.cls_nq[c]) {
-  case 2: ens_items<NT, 2, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
-  case 3: ens_items<NT, 3, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
-  case 4: ens_items<NT, 4, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
-  case 5: ens_items<NT, 5, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
-  case 6: ens_items<NT, 6, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
-  case 7: ens_items<NT, 7, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
-  case 8: ens_items<NT, 8, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
-  case 9: ens_items<NT, 9, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
-  case 10: ens_items<NT, 10, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
-  case 11: ens_items<NT, 11, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
-  case 12: ens_items<NT, 12, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
-  case 13: ens_items<NT, 13, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
+  case 2: ens_items<NT, 2, ORDER, FMA, FACT, CV>(A, first, count, epb, lb, nb, smem); break;
+  case 3: ens_items<NT, 3, ORDER, FMA, FACT, CV>(A, first, count, epb, lb, nb, smem); break;
+  case 4: ens_items<NT, 4, ORDER, FMA, FACT, CV>(A, first, count, epb, lb, nb, smem); break;
+  case 5: ens_items<NT, 5, ORDER, FMA, FACT, CV>(A, first, count, epb, lb, nb, smem); break;
+  case 6: ens_items<NT, 6, ORDER, FMA, FACT, CV>(A, first, count, epb, lb, nb, smem); break;
+  case 7: ens_items<NT, 7, ORDER, FMA, FACT, CV>(A, first, count, epb, lb, nb, smem); break;
+  case 8: ens_items<NT, 8, ORDER, FMA, FACT, CV>(A, first, count, epb, lb, nb, smem); break;
+  case 9: ens_items<NT, 9, ORDER, FMA, FACT, CV>(A, first, count, epb, lb, nb, smem); break;
+  case 10: ens_items<NT, 10, ORDER, FMA, FACT, CV>(A, first, count, epb, lb, nb, smem); break;
+  case 11: ens_items<NT, 11, ORDER, FMA, FACT, CV>(A, first, count, epb, lb, nb, smem); break;
+  case 12: ens_items<NT, 12, ORDER, FMA, FACT, CV>(A, first, count, epb, lb, nb, smem); break;
+  case 13: ens_items<NT, 13, ORDER, FMA, FACT, CV>(A, first, count, epb, lb, nb, smem); break;
   default:
     if constexpr (NT >= 256) {
       switch (A.cls_nq[c]) {
-      case 14: ens_items<NT, 14, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
-      case 15: ens_items<NT, 15, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
-      default: ens_items<NT, 16, ORDER, FMA, FACT, CV>(A, first, count, lb, nb, smem); break;
+      case 14: ens_items<NT, 14, ORDER, FMA, FACT, CV>(A, first, count, epb, lb, nb, smem); break;
+      case 15: ens_items<NT, 15, ORDER, FMA, FACT, CV>(A, first, count, epb, lb, nb, smem); break;
+      default: ens_items<NT, 16, ORDER, FMA, FACT, CV>(A, first, count, epb, lb, nb, smem); break;
       }
     }
     break;
@@ -231,9 +235,13 @@ struct dgk_ensemble {
   StageParams *dparams = nullptr;
   int2 *ditems = nullptr;
   int nitems = 0;
-  // classes of equal order and the split of the grid between them
+  std::vector<int> order;  // instances by increasing order
+  int max_nq = 0;
+  // classes of equal order, their elements per item and the split of the
+  // grid between them
   int ncls = 0;
   int cls_nq[kEnsMaxClass] = {};
+  int cls_epb[kEnsMaxClass] = {};
   int cls_item[kEnsMaxClass + 1] = {};
   int cls_block[kEnsMaxClass + 1] = {};
   double cls_weight[kEnsMaxClass] = {};  // DOF, or measured stage time once tuned
@@ -309,7 +317,10 @@ EnsArgs ens_args(const dgk_ensemble *s, int stage) {
   A.items = s->ditems;
   A.params = s->dparams + ((size_t)(9 * s->cur + 3 * s->c0) + stage) * s->n;
   A.ncls = s->ncls;
-  for (int c = 0; c < s->ncls; ++c) A.cls_nq[c] = s->cls_nq[c];
+  for (int c = 0; c < s->ncls; ++c) {
+    A.cls_nq[c] = s->cls_nq[c];
+    A.cls_epb[c] = s->cls_epb[c];
+  }
   for (int c = 0; c <= s->ncls; ++c) {
     A.cls_item[c] = s->cls_item[c];
     A.cls_block[c] = s->cls_block[c];
@@ -339,6 +350,16 @@ int ens_unit_cap(const dgk_ensemble *s, int c) {
   return std::max(1, (s->cls_item[c + 1] - s->cls_item[c] + kNumXcd - 1) / kNumXcd);
 }
 
+// The grid split as units of eight workgroups per class.
+void ens_set_units(dgk_ensemble *s, const int *u) {
+  s->cls_block[0] = 0;
+  for (int c = 0; c < s->ncls; ++c) s->cls_block[c + 1] = s->cls_block[c] + kNumXcd * u[c];
+  s->grid = s->cls_block[s->ncls];
+}
+void ens_get_units(const dgk_ensemble *s, int *u) {
+  for (int c = 0; c < s->ncls; ++c) u[c] = (s->cls_block[c + 1] - s->cls_block[c]) / kNumXcd;
+}
+
 // Splits the resident workgroups between the classes in proportion to
 // cls_weight (largest remaining weight per unit first), at least one unit
 // and at most ens_unit_cap units per class.
@@ -365,10 +386,73 @@ void ens_partition(dgk_ensemble *s) {
     ++u[best];
     ++used;
   }
-  s->cls_block[0] = 0;
-  for (int c = 0; c < s->ncls; ++c) s->cls_block[c + 1] = s->cls_block[c] + kNumXcd * u[c];
-  s->grid = s->cls_block[s->ncls];
+  ens_set_units(s, u);
 }
+
+// The tile shape: nt threads per workgroup and epb[c] elements per item of
+// class c (0: as many as fill the workgroup).  Checked before anything
+// changes; the work items, the kernel, its LDS and the resident workgroups
+// then follow, and the grid is split by the present class weights.  The
+// caller has waited for the stream (launches in flight read the items).
+int ens_check_shape(const dgk_ensemble *s, int nt, const int *epb) {
+  if (nt != kEnsNarrow && nt != kEnsWide)
+    return fail(DGK_ERR_ARG, "threads per workgroup %d: %d or %d", nt, kEnsNarrow, kEnsWide);
+  if (nt < ens_threads(s->max_nq))
+    return fail(DGK_ERR_ARG, "Nq %d (%d lines an element) needs %d threads per workgroup",
+                s->max_nq, s->max_nq * s->max_nq, kEnsWide);
+  for (int c = 0; epb && c < s->ncls; ++c)
+    if (epb[c] < 0 || epb[c] > ens_item_elems(s->cls_nq[c], nt))
+      return fail(DGK_ERR_ARG, "class %d (Nq %d): %d elements per item outside 0..%d", c,
+                  s->cls_nq[c], epb[c], ens_item_elems(s->cls_nq[c], nt));
+  return DGK_OK;
+}
+
+int ens_shape(dgk_ensemble *s, int nt, const int *epb) {
+  int rc = ens_check_shape(s, nt, epb);
+  if (rc) return rc;
+  s->nt = nt;
+  size_t lds = 0;
+  for (int c = 0; c < s->ncls; ++c) {
+    const int most = ens_item_elems(s->cls_nq[c], nt);
+    s->cls_epb[c] = epb && epb[c] ? epb[c] : most;
+    lds = std::max(lds, s->cls_epb[c] * ens_elem_lds(s->cls_nq[c]));
+  }
+  std::vector<int2> items;
+  int c = -1;
+  for (int i : s->order) {
+    while (c < 0 || s->cls_nq[c] != s->inst[i].nq) s->cls_item[++c] = (int)items.size();
+    for (long long e = 0; e < s->inst[i].ne; e += s->cls_epb[c])
+      items.push_back(make_int2(i, (int)e));
+  }
+  s->cls_item[s->ncls] = (int)items.size();
+  s->nitems = (int)items.size();
+  s->lds = lds;
+  HIPOK(hipMemcpy(s->ditems, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice));
+  const int f = s->flags;
+  s->fn = pick_ens(nt, (f & DGK_FLAG_MIXED) != 0, (f & DGK_FLAG_FMA) != 0,
+                   (f & DGK_FLAG_FACTORED_LIFT) != 0, (f & DGK_FLAG_CONST_VELOCITY) != 0);
+  int dev = 0, ncu = 0, per_cu = 0;
+  if (hipFuncSetAttribute((const void *)s->fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                          (int)s->lds) != hipSuccess ||
+      hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)s->fn, nt, s->lds) !=
+          hipSuccess)
+    return fail(DGK_ERR_HIP, "ensemble kernel configuration failed");
+  s->slots = std::max(1, per_cu) * ncu / kNumXcd * kNumXcd;
+  ens_drop_graphs(s);
+  ens_partition(s);
+  return DGK_OK;
+}
+
+// Class weights before any measurement: the DOF of each class.
+void ens_dof(const dgk_ensemble *s, double *w) {
+  for (int c = 0; c < s->ncls; ++c) w[c] = 0.0;
+  for (int i = 0; i < s->n; ++i)
+    for (int c = 0; c < s->ncls; ++c)
+      if (s->cls_nq[c] == s->inst[i].nq) w[c] += (double)s->inst[i].ne * s->np[i];
+}
+void ens_dof_weights(dgk_ensemble *s) { ens_dof(s, s->cls_weight); }
 
 template <class T>
 int ens_alloc(T **p, size_t n) {
@@ -502,36 +586,20 @@ int dgk_ensemble_create(dgk_ensemble **out, int n, const int32_t *p, const int32
           }
   }
   s->total = off;
-  // work items sorted by order, then instance, then element
-  std::vector<int2> items;
-  std::vector<int> order(n);
-  for (int i = 0; i < n; ++i) order[i] = i;
-  std::stable_sort(order.begin(), order.end(),
+  // classes of equal order; work items (instance, first element) sorted by
+  // order, then instance, then element, built by ens_shape
+  s->order.resize(n);
+  for (int i = 0; i < n; ++i) s->order[i] = i;
+  std::stable_sort(s->order.begin(), s->order.end(),
                    [&](int a, int b) { return s->inst[a].nq < s->inst[b].nq; });
-  int max_nq = 0;
-  for (int i = 0; i < n; ++i) max_nq = std::max(max_nq, s->inst[i].nq);
-  s->nt = ens_threads(max_nq);
-  size_t lds = 0;
-  for (int i : order) {
-    const int per = ens_item_elems(s->inst[i].nq, s->nt);
-    for (long long e = 0; e < s->inst[i].ne; e += per) items.push_back(make_int2(i, (int)e));
-    lds = std::max(lds, ens_item_lds(s->inst[i].nq, s->nt));
+  long long nelem = 0;
+  for (int i : s->order) {
+    const int nq = s->inst[i].nq;
+    if (!s->ncls || s->cls_nq[s->ncls - 1] != nq) s->cls_nq[s->ncls++] = nq;
+    s->max_nq = std::max(s->max_nq, nq);
+    nelem += s->inst[i].ne;
   }
-  s->nitems = (int)items.size();
-  s->lds = lds;
-  for (size_t t = 0; t < items.size(); ++t) {
-    const int nq = s->inst[items[t].x].nq;
-    if (!s->ncls || s->cls_nq[s->ncls - 1] != nq) {
-      s->cls_nq[s->ncls] = nq;
-      s->cls_item[s->ncls] = (int)t;
-      s->cls_weight[s->ncls] = 0.0;
-      ++s->ncls;
-    }
-  }
-  s->cls_item[s->ncls] = (int)items.size();
-  for (int i = 0; i < n; ++i)
-    for (int c = 0; c < s->ncls; ++c)
-      if (s->cls_nq[c] == s->inst[i].nq) s->cls_weight[c] += (double)s->inst[i].ne * s->np[i];
+  ens_dof_weights(s);
   if (!rc) rc = ens_alloc(&s->q[0], off);
   if (!rc) rc = ens_alloc(&s->q[1], off);
   if (!rc) rc = ens_alloc(&s->q[2], off);
@@ -541,32 +609,16 @@ int dgk_ensemble_create(dgk_ensemble **out, int n, const int32_t *p, const int32
   if (!rc) rc = ens_alloc(&s->nbr, nbr.size());
   if (!rc) rc = ens_alloc(&s->dinst, (size_t)n);
   if (!rc) rc = ens_alloc(&s->dops, hops.size());
-  if (!rc) rc = ens_alloc(&s->ditems, items.size());
+  if (!rc) rc = ens_alloc(&s->ditems, (size_t)nelem);  // one element per item at most
   if (!rc) rc = ens_alloc(&s->dparams, (size_t)27 * n);
   if (!rc && (hipMemcpy(s->nbr, nbr.data(), nbr.size() * sizeof(int32_t), hipMemcpyHostToDevice) ||
               hipMemcpy(s->dops, hops.data(), hops.size() * sizeof(EnsOps), hipMemcpyHostToDevice) ||
-              hipMemcpy(s->ditems, items.data(), items.size() * sizeof(int2),
-                        hipMemcpyHostToDevice) ||
               hipMemcpy(s->dinst, s->inst.data(), n * sizeof(EnsInst), hipMemcpyHostToDevice)))
     rc = fail(DGK_ERR_HIP, "ensemble table upload failed");
   if (!rc && hipMemset(s->q[2], 0, off * sizeof(double)) != hipSuccess)
     rc = fail(DGK_ERR_HIP, "hipMemset failed");
   if (!rc) rc = ens_upload_params(s);
-  if (!rc) {
-    s->fn = pick_ens(s->nt, (flags & DGK_FLAG_MIXED) != 0, (flags & DGK_FLAG_FMA) != 0,
-                     (flags & DGK_FLAG_FACTORED_LIFT) != 0,
-                     (flags & DGK_FLAG_CONST_VELOCITY) != 0);
-    int dev = 0, ncu = 0, per_cu = 0;
-    if (hipFuncSetAttribute((const void *)s->fn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)s->lds) != hipSuccess ||
-        hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)s->fn, s->nt,
-                                                     s->lds) != hipSuccess)
-      rc = fail(DGK_ERR_HIP, "ensemble kernel configuration failed");
-    s->slots = std::max(1, per_cu) * ncu / kNumXcd * kNumXcd;
-    ens_partition(s);
-  }
+  if (!rc) rc = ens_shape(s, ens_threads(s->max_nq), nullptr);
   if (rc) {
     dgk_ensemble_destroy(s);
     return rc;
@@ -682,6 +734,155 @@ int dgk_ensemble_time_stages(dgk_ensemble *s, int nstage, double *ms_per_stage) 
   return DGK_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Average time of nrep launches of A (after one untimed launch).
+int ens_time_args(dgk_ensemble *s, const EnsArgs &A, int nrep, hipEvent_t e0, hipEvent_t e1,
+                  double *ms) {
+  int rc = ens_launch_args(s, A);
+  if (rc) return rc;
+  HIPOK(hipEventRecord(e0, s->stream));
+  for (int r = 0; r < nrep && !rc; ++r) rc = ens_launch_args(s, A);
+  if (rc) return rc;
+  float f = 0.f;
+  HIPOK(hipEventRecord(e1, s->stream));
+  HIPOK(hipEventSynchronize(e1));
+  HIPOK(hipEventElapsedTime(&f, e0, e1));
+  *ms = (double)f / nrep;
+  return DGK_OK;
+}
+
+// Stage 1 of class c alone on as much of the grid as it can use: reads q and
+// writes the buffer that holds neither q nor q0, so the state is left as it
+// was.  *wg_ms: the time times the workgroups it had, the share of the grid
+// the class needs for an equal finish time.
+int ens_time_class(dgk_ensemble *s, int c, int nrep, hipEvent_t e0, hipEvent_t e1, double *ms,
+                   double *wg_ms) {
+  EnsArgs A = ens_args(s, 0);
+  const int u = std::min(std::max(s->slots / kNumXcd, 1), ens_unit_cap(s, c));
+  A.ncls = 1;
+  A.cls_nq[0] = s->cls_nq[c];
+  A.cls_epb[0] = s->cls_epb[c];
+  A.cls_item[0] = s->cls_item[c];
+  A.cls_item[1] = s->cls_item[c + 1];
+  A.cls_block[0] = 0;
+  A.cls_block[1] = kNumXcd * u;
+  int rc = ens_time_args(s, A, nrep, e0, e1, ms);
+  *wg_ms = *ms * u;
+  return rc;
+}
+
+// The tile-shape search of dgk_ensemble_tune.  For each workgroup width
+// that the largest order allows, each class in turn tries its candidate
+// elements per item (all that fill the workgroup, about half, one) with the
+// other classes at their choice so far, and keeps the one with the least
+// workgroup-time (time alone times the workgroups it had).  The grid is then
+// split between the classes in proportion to those workgroup-times, to their
+// DOF, and to the geometric mean of the two, the whole stage is timed for
+// each, and the fastest width and split are kept.  Last, one unit of eight
+// workgroups at a time moves between the largest class and each other one
+// while the whole stage gets faster (classes timed alone misjudge what they
+// take from each other when they share the chip).
+int ens_tune(dgk_ensemble *s, int nrep, double *ms_per_class, hipEvent_t e0, hipEvent_t e1) {
+  int widths[2], nw = 0;
+  if (s->max_nq <= 13) widths[nw++] = kEnsNarrow;
+  widths[nw++] = kEnsWide;
+  double best_stage = 0.0;
+  int best_nt = 0, best_epb[kEnsMaxClass] = {}, best_u[kEnsMaxClass] = {};
+  double best_ms[kEnsMaxClass] = {};
+  double dof[kEnsMaxClass], dsum = 0.0;
+  ens_dof(s, dof);
+  for (int c = 0; c < s->ncls; ++c) dsum += dof[c];
+  for (int k = 0; k < nw; ++k) {
+    const int nt = widths[k];
+    int epb[kEnsMaxClass];
+    double w[kEnsMaxClass], ms[kEnsMaxClass];
+    for (int c = 0; c < s->ncls; ++c) epb[c] = ens_item_elems(s->cls_nq[c], nt);
+    for (int c = 0; c < s->ncls; ++c) {
+      const int most = epb[c];
+      const int cand[3] = {most, (most + 1) / 2, 1};
+      double bw = 0.0, bms = 0.0;
+      int be = most;
+      for (int j = 0; j < 3; ++j) {
+        if (j && (cand[j] == cand[j - 1] || cand[j] == most)) continue;
+        epb[c] = cand[j];
+        double t = 0.0, wt = 0.0;
+        int rc = ens_shape(s, nt, epb);
+        if (!rc) rc = ens_time_class(s, c, nrep, e0, e1, &t, &wt);
+        if (rc) return rc;
+        if (j == 0 || wt < bw) {
+          bw = wt;
+          bms = t;
+          be = cand[j];
+        }
+      }
+      epb[c] = be;
+      w[c] = bw;
+      ms[c] = bms;
+    }
+    int rc = ens_shape(s, nt, epb);
+    if (rc) return rc;
+    double wsum = 0.0;
+    for (int c = 0; c < s->ncls; ++c) wsum += w[c];
+    for (int split = 0; split < 3; ++split) {
+      for (int c = 0; c < s->ncls; ++c) {
+        const double a = w[c] / wsum, b = dof[c] / dsum;
+        s->cls_weight[c] = split == 0 ? a : split == 1 ? b : std::sqrt(a * b);
+      }
+      ens_partition(s);
+      double stage = 0.0;
+      if ((rc = ens_time_args(s, ens_args(s, 0), nrep, e0, e1, &stage))) return rc;
+      if ((k == 0 && split == 0) || stage < best_stage) {
+        best_stage = stage;
+        best_nt = nt;
+        ens_get_units(s, best_u);
+        for (int c = 0; c < s->ncls; ++c) {
+          best_epb[c] = epb[c];
+          best_ms[c] = ms[c];
+        }
+      }
+    }
+  }
+  int rc = ens_shape(s, best_nt, best_epb);
+  if (rc) return rc;
+  ens_set_units(s, best_u);
+  int big = 0;
+  for (int c = 1; c < s->ncls; ++c)
+    if (best_u[c] > best_u[big]) big = c;
+  for (int c = 0; c < s->ncls; ++c) {
+    if (c == big) continue;
+    for (int dir = 1; dir >= -1; dir -= 2) {
+      for (int step = 0; step < 4; ++step) {  // while it pays, up to four units
+        int u[kEnsMaxClass];
+        for (int d = 0; d < s->ncls; ++d) u[d] = best_u[d];
+        u[c] += dir;
+        u[big] -= dir;
+        if (u[c] < 1 || u[big] < 1 || u[c] > ens_unit_cap(s, c) || u[big] > ens_unit_cap(s, big))
+          break;
+        ens_set_units(s, u);
+        double stage = 0.0;
+        if ((rc = ens_time_args(s, ens_args(s, 0), nrep, e0, e1, &stage))) return rc;
+        if (!(stage < best_stage * 0.998)) break;
+        best_stage = stage;
+        for (int d = 0; d < s->ncls; ++d) best_u[d] = u[d];
+      }
+    }
+  }
+  // the split kept as weights that ens_partition maps back to these units
+  for (int c = 0; c < s->ncls; ++c) {
+    s->cls_weight[c] = best_u[c];
+    if (ms_per_class) ms_per_class[c] = best_ms[c];
+  }
+  ens_set_units(s, best_u);
+  return DGK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int dgk_ensemble_tune(dgk_ensemble *s, int nrep, double *ms_per_class) {
   if (!s || nrep < 1) return fail(DGK_ERR_ARG, "bad argument");
   if (!s->fields_set) return fail(DGK_ERR_STATE, "no fields set");
@@ -690,40 +891,30 @@ int dgk_ensemble_tune(dgk_ensemble *s, int nrep, double *ms_per_class) {
   int rc = DGK_OK;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
     rc = fail(DGK_ERR_HIP, "ensemble tune: hipEventCreate failed");
-  const int units = std::max(s->slots / kNumXcd, 1);
-  for (int c = 0; c < s->ncls && !rc; ++c) {
-    // stage 1 of class c alone on the whole grid: reads q and writes the
-    // buffer that holds neither q nor q0, so the state is left as it was
-    EnsArgs A = ens_args(s, 0);
-    const int u = std::min(units, ens_unit_cap(s, c));
-    A.ncls = 1;
-    A.cls_nq[0] = s->cls_nq[c];
-    A.cls_item[0] = s->cls_item[c];
-    A.cls_item[1] = s->cls_item[c + 1];
-    A.cls_block[0] = 0;
-    A.cls_block[1] = kNumXcd * u;
-    if ((rc = ens_launch_args(s, A))) break;
-    if (hipEventRecord(e0, s->stream) != hipSuccess) rc = fail(DGK_ERR_HIP, "hipEventRecord");
-    for (int r = 0; r < nrep && !rc; ++r) rc = ens_launch_args(s, A);
-    float ms = 0.f;
-    if (!rc && (hipEventRecord(e1, s->stream) != hipSuccess ||
-                hipEventSynchronize(e1) != hipSuccess ||
-                hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
-      rc = fail(DGK_ERR_HIP, "event timing failed");
-    if (rc) break;
-    const double t = (double)ms / nrep;
-    if (ms_per_class) ms_per_class[c] = t;
-    // workgroups needed for an equal finish time scale with the time the
-    // class takes alone times the workgroups it had then
-    s->cls_weight[c] = t * u;
-  }
+  if (!rc) rc = ens_tune(s, nrep, ms_per_class, e0, e1);
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
-  if (rc) return rc;
-  ens_drop_graphs(s);
-  ens_partition(s);
+  if (rc) {  // back to the untuned shape
+    ens_dof_weights(s);
+    s->tuned = false;
+    (void)ens_shape(s, ens_threads(s->max_nq), nullptr);
+    return rc;
+  }
   s->tuned = true;
   return DGK_OK;
+}
+
+int dgk_ensemble_set_shape(dgk_ensemble *s, int threads, const int32_t *elems_per_item) {
+  if (!s) return fail(DGK_ERR_ARG, "null ensemble");
+  const int nt = threads ? threads : ens_threads(s->max_nq);
+  int epb[kEnsMaxClass];
+  for (int c = 0; c < s->ncls; ++c) epb[c] = elems_per_item ? elems_per_item[c] : 0;
+  int rc = ens_check_shape(s, nt, epb);
+  if (rc) return rc;
+  HIPOK(hipStreamSynchronize(s->stream));
+  ens_dof_weights(s);
+  s->tuned = false;
+  return ens_shape(s, nt, epb);
 }
 
 int dgk_ensemble_diagnostics(dgk_ensemble *s, int i, const double *w1D, const double *ref,
@@ -752,6 +943,8 @@ int dgk_ensemble_query(dgk_ensemble *s, int what, int64_t *value) {
   default:
     if (what >= 16 && what < 16 + s->ncls) {  // Nq of class c = what-16
       *value = s->cls_nq[what - 16];
+    } else if (what >= 48 && what < 48 + s->ncls) {  // elements per item of class c
+      *value = s->cls_epb[what - 48];
     } else if (what >= 32 && what < 32 + s->ncls) {  // workgroups of class c
       *value = s->cls_block[what - 32 + 1] - s->cls_block[what - 32];
     } else {

@@ -38,7 +38,7 @@ EXPORTS = (
     "dgk_lift_hexahedral_batched", "dgk_ensemble_create", "dgk_ensemble_destroy",
     "dgk_ensemble_set_fields", "dgk_ensemble_get_field", "dgk_ensemble_set_dt",
     "dgk_ensemble_step", "dgk_ensemble_run", "dgk_ensemble_time_stages",
-    "dgk_ensemble_tune", "dgk_ensemble_query", "dgk_autotune", "dgk_diagnostics",
+    "dgk_ensemble_tune", "dgk_ensemble_query", "dgk_ensemble_set_shape", "dgk_autotune", "dgk_diagnostics",
     "dgk_diagnostics_noalloc",
     "dgk_ensemble_diagnostics",
     "dgk_matvec_dir_batched", "dgk_gradlike_dirXYZ_batched", "dgk_divlike_dirXYZ_var5_batched",
@@ -108,6 +108,7 @@ def lib():
         "dgk_ensemble_time_stages": ([vp, i, vp], i),
         "dgk_ensemble_tune": ([vp, i, vp], i),
         "dgk_ensemble_query": ([vp, i, vp], i),
+        "dgk_ensemble_set_shape": ([vp, i, vp], i),
         "dgk_ensemble_diagnostics": ([vp, i, vp, vp, vp], i),
     }
     for name, (args, res) in sig.items():
@@ -457,6 +458,21 @@ class Ensemble:
         v = C.c_longlong(0)
         _check(lib().dgk_ensemble_query(self._h, what, C.byref(v)))
         return v.value
+
+    def set_shape(self, threads=0, elems_per_item=None):
+        """Tile shape by hand: threads per workgroup (192, 256; 0 default)
+        and elements per item of each class (0 or None: as many as fit)."""
+        e = None
+        if elems_per_item is not None:
+            e = np.ascontiguousarray(elems_per_item, dtype=np.int32)
+            if e.shape != (self.query(4),):
+                raise ValueError("one entry per order class")
+        _check(lib().dgk_ensemble_set_shape(self._h, threads, None if e is None else _p(e, np.int32)))
+
+    @property
+    def shape(self):
+        """(threads per workgroup, [elements per item of each class])."""
+        return self.query(6), [self.query(48 + c) for c in range(self.query(4))]
 
     def diagnostics(self, i, x1D, ref=None):
         """Solver.diagnostics for instance i (ref: a device address of a

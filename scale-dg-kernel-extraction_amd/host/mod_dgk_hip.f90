@@ -36,6 +36,7 @@ module mod_dgk_hip
   public :: dgk_ensemble_create, dgk_ensemble_destroy, dgk_ensemble_set_fields
   public :: dgk_ensemble_get_field, dgk_ensemble_set_dt, dgk_ensemble_step, dgk_ensemble_run
   public :: dgk_ensemble_time_stages, dgk_ensemble_tune, dgk_ensemble_query
+  public :: dgk_ensemble_set_shape
   public :: dgk_ensemble_diagnostics
 
   interface
@@ -213,6 +214,16 @@ module mod_dgk_hip
       type(c_ptr), value :: ens
       integer(c_int), value :: nrep
       real(c_double), intent(out) :: ms(*)
+    end function
+
+    ! tile shape by hand: threads 192 or 256 (0: default), elements per item
+    ! of each order class (0: as many as fill the workgroup)
+    integer(c_int) function dgk_ensemble_set_shape(ens, threads, epb) &
+         bind(C, name="dgk_ensemble_set_shape")
+      import :: c_int, c_int32_t, c_ptr
+      type(c_ptr), value :: ens
+      integer(c_int), value :: threads
+      integer(c_int32_t), intent(in) :: epb(*)
     end function
 
     ! dgk_diagnostics for instance i (0-based) of the ensemble

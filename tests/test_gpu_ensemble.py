@@ -138,6 +138,9 @@ def test_ensemble_tune_keeps_state_and_bits(gpu, small):
     cls = b.classes
     assert [c[0] for c in cls] == [2, 4, 6, 8, 12]
     assert all(wg > 0 and wg % 8 == 0 for _, wg in cls)
+    nt, epb = b.shape  # the tile shape the tuning chose
+    most = [nt // (nq * nq) for nq, _ in cls]
+    assert nt in (192, 256) and all(1 <= e <= m for e, m in zip(epb, most)), (nt, epb)
     for i in range(len(small)):  # tuning left the state alone
         assert np.array_equal(b.get_field(i), small[i]["q"])
     a.run(2)
@@ -146,6 +149,62 @@ def test_ensemble_tune_keeps_state_and_bits(gpu, small):
         assert np.array_equal(a.get_field(i), b.get_field(i)), i
     a.close()
     b.close()
+
+
+# most elements per item of Nq 2, 4, 6, 8, 12 at 192 and 256 threads
+MOST = {192: [48, 12, 5, 3, 1], 256: [64, 16, 7, 4, 1]}
+
+
+@pytest.mark.parametrize("flags", [0, dgk.FLAG_FMA | dgk.FLAG_FACTORED_LIFT |
+                                   dgk.FLAG_CONST_VELOCITY, dgk.FLAG_MIXED])
+def test_ensemble_tile_shapes_keep_bits(gpu, small, flags):
+    """Every tile shape gives the bits of the default one: both workgroup
+    widths, each class at the most, about half and one element per item,
+    and uneven choices (ragged items at every class)."""
+    ref = _ensemble(small, flags)
+    assert ref.shape == (192, MOST[192])
+    ref.run(2)
+    want = [ref.get_field(i) for i in range(len(small))]
+    ref.close()
+    for nt in (192, 256):
+        most = MOST[nt]
+        for name, epb in (("most", most), ("half", [(m + 1) // 2 for m in most]),
+                          ("one", [1] * 5), ("uneven", [max(1, m - 1) for m in most])):
+            say(f"[ens] shape flags={flags} threads={nt} {name} {epb}")
+            ens = _ensemble(small, flags)
+            ens.set_shape(nt, epb)
+            assert ens.shape == (nt, epb)
+            assert ens.query(1) == sum(-(-int(np.prod(s["ne"])) // epb[[1, 3, 5, 7, 11].index(
+                s["p"])]) for s in small)
+            assert all(wg > 0 and wg % 8 == 0 for _, wg in ens.classes)
+            ens.run(2)
+            for i in range(len(small)):
+                assert np.array_equal(ens.get_field(i), want[i]), (nt, name, i)
+            ens.close()
+
+
+def test_ensemble_tile_shape_errors(gpu, small):
+    """Shapes outside what the workgroup holds are refused and leave the
+    shape as it was; an instance of order 13 rules out 192 threads."""
+    ens = _ensemble(small)
+    ens.set_shape(256, [0, 8, 0, 2, 0])
+    assert ens.shape == (256, [64, 8, 7, 2, 1])
+    for nt, epb in ((128, None), (512, None), (192, [49, 1, 1, 1, 1]), (256, [1, 1, 1, 1, 2]),
+                    (192, [1, 1, 1, 1, -1])):
+        with pytest.raises(dgk.DGKError):
+            ens.set_shape(nt, epb)
+        assert ens.shape == (256, [64, 8, 7, 2, 1])
+    with pytest.raises(ValueError):
+        ens.set_shape(192, [1, 1])
+    ens.set_shape()
+    assert ens.shape == (192, MOST[192])
+    ens.close()
+    inst = _setup([dict(p=13, ne=(2, 2, 2)), dict(p=3, ne=(2, 2, 2))], 7)
+    ens = _ensemble(inst)
+    assert ens.shape == (256, [16, 1])
+    with pytest.raises(dgk.DGKError):
+        ens.set_shape(192)
+    ens.close()
 
 
 def test_ensemble_step_graph_vs_run(gpu, small):
