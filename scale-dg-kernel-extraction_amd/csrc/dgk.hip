@@ -477,6 +477,7 @@ struct dgk_solver {
   double *u = nullptr, *v = nullptr, *w = nullptr, *dqdt = nullptr;
   double *D = nullptr, *Lift = nullptr, *lfac = nullptr, *Drow = nullptr, *LiftP = nullptr;
   double *Drow32 = nullptr;  // FP32 split operator and line-fit tables (mixed precision)
+  bool lift_packed = false;  // LiftP holds the delta-packed Lift (pack_lift_deltas)
   int32_t *nbr = nullptr, *vmapM = nullptr, *vmapP = nullptr, *halo = nullptr;
   double *nrm = nullptr, *fscale = nullptr, *escale = nullptr;
   double fs[6] = {0, 0, 0, 0, 0, 0}, es[3] = {0, 0, 0};
@@ -559,12 +560,20 @@ template <class T>
 int upload(T **dst, const T *src, size_t n, hipStream_t st);
 
 int configure_launch(dgk_solver *s) {
-  const bool use_pencil =
-      s->structured && !(s->flags & DGK_FLAG_NODE_KERNEL) && pencil_nq(s->Nq);
+  const bool use_pencil = s->structured && !(s->flags & DGK_FLAG_NODE_KERNEL) &&
+                          pencil_nq(s->Nq) &&
+                          (s->lift_packed || (s->flags & DGK_FLAG_FACTORED_LIFT) ||
+                           s->order == kMixed);
   const int var = current_variant(s);
   if (s->order == kMixed && !use_pencil)
     return fail(DGK_ERR_ARG, "DGK_FLAG_MIXED needs the pencil kernel (compressed "
                              "connectivity, no DGK_FLAG_NODE_KERNEL) without DGK_FLAG_FMA");
+  if ((s->flags & DGK_FLAG_CONST_VELOCITY) && !use_pencil && !s->lift_packed &&
+      !(s->flags & DGK_FLAG_FACTORED_LIFT))
+    return fail(DGK_ERR_ARG, "DGK_FLAG_CONST_VELOCITY needs the pencil kernel, whose exact lift "
+                             "needs a Lift whose values along each face's normal index lie within "
+                             "255 units in the last place of one another: add "
+                             "DGK_FLAG_FACTORED_LIFT");
   if ((s->flags & DGK_FLAG_CONST_VELOCITY) && !use_pencil)
     return fail(DGK_ERR_ARG, "DGK_FLAG_CONST_VELOCITY needs the pencil kernel (compressed "
                              "connectivity, no DGK_FLAG_NODE_KERNEL)");
@@ -794,20 +803,16 @@ int finish_create(dgk_solver *s, const double *D1D, const double *Lift) {
   const size_t nq2 = (size_t)Nq * Nq, nlift = (size_t)6 * Np;
   if ((rc = dalloc(&s->D, nq2)) || (rc = dalloc(&s->Lift, nlift)) ||
       (rc = dalloc(&s->lfac, 2 * (size_t)Nq)) || (rc = dalloc(&s->Drow, nq2)) ||
-      (rc = dalloc(&s->LiftP, nlift)) || (rc = dalloc(&s->Drow32, nq2 + 3 * Nq)))
+      (rc = dalloc(&s->LiftP, lift_delta_doubles(Nq))) ||
+      (rc = dalloc(&s->Drow32, nq2 + 3 * Nq)))
     return rc;
-  // pair-packed Lift for the pencil kernel: line pl = i + Nq*j holds the
-  // 6*Nq values v = 6k + f, Lift(i,j,k,f); pair m of line pl at
-  // [(m*Nq^2 + pl)*2 + {0,1}]
+  // the delta-packed Lift of the pencil kernels (without it, an exact lift
+  // takes the node kernel)
   {
-    const int Nfp = Nq * Nq;
-    std::vector<double> lp(nlift);
-    for (int pl = 0; pl < Nfp; ++pl)
-      for (int v = 0; v < 6 * Nq; ++v) {
-        const int k = v / 6, f = v % 6, m = v / 2;
-        lp[((size_t)m * Nfp + pl) * 2 + (v & 1)] = Lift[pl + (size_t)Nfp * k + (size_t)Np * f];
-      }
-    HIPOK(hipMemcpy(s->LiftP, lp.data(), nlift * sizeof(double), hipMemcpyHostToDevice));
+    std::vector<double> lp(lift_delta_doubles(Nq));
+    s->lift_packed = pack_lift_deltas(Nq, Lift, lp.data());
+    if (s->lift_packed)
+      HIPOK(hipMemcpy(s->LiftP, lp.data(), lp.size() * sizeof(double), hipMemcpyHostToDevice));
     s->ops.LiftP = s->LiftP;
   }
   HIPOK(hipMemcpy(s->D, D1D, nq2 * sizeof(double), hipMemcpyHostToDevice));

@@ -523,17 +523,18 @@ int dgk_ensemble_create(dgk_ensemble **out, int n, const int32_t *p, const int32
   std::vector<EnsOps> &hops = s->hops;
   for (int i = 0; i < n && !rc; ++i) {
     if (slot.count(p[i])) continue;
-    const int Nq = p[i] + 1, Np = Nq * Nq * Nq, Nfp = Nq * Nq;
+    const int Nq = p[i] + 1, Np = Nq * Nq * Nq;
     std::vector<double> x1D(Nq), D(Nq * Nq), L(6 * (size_t)Np);
     if ((rc = dgk_read_operator(operator_dir, p[i], x1D.data(), D.data(), L.data()))) break;
-    std::vector<double> Drow(Nq * Nq), LiftP(L.size()), lf(2 * Nq);
+    std::vector<double> Drow(Nq * Nq), LiftP(lift_delta_doubles(Nq)), lf(2 * Nq);
     for (int k = 0; k < Nq; ++k)
       for (int l = 0; l < Nq; ++l) Drow[k * Nq + l] = D[k + l * Nq];
-    for (int pl = 0; pl < Nfp; ++pl)
-      for (int v = 0; v < 6 * Nq; ++v) {
-        const int k = v / 6, f = v % 6, m = v / 2;
-        LiftP[((size_t)m * Nfp + pl) * 2 + (v & 1)] = L[pl + (size_t)Nfp * k + (size_t)Np * f];
-      }
+    if (!pack_lift_deltas(Nq, L.data(), LiftP.data()) && !(flags & DGK_FLAG_FACTORED_LIFT)) {
+      rc = fail(DGK_ERR_ARG, "PolyOrder %d: the exact lift needs a Lift whose values along each "
+                "face's normal index lie within 255 units in the last place of one another; "
+                "add DGK_FLAG_FACTORED_LIFT", p[i]);
+      break;
+    }
     for (int t = 0; t < Nq; ++t) {
       lf[t] = L[t + (size_t)Np * 3];
       lf[Nq + t] = L[t + (size_t)Np * 1];

@@ -151,6 +151,56 @@ void split_tables(int Nq, const double *drow, double *d32) {
   T[Nq - 1] = Nq >= 4 ? 1.0 / p3(t[b]) : 0.0;
 }
 
+
+// Face f's normal index at node (i,j,k) of an element: j for -y and +y
+// (f = 0, 2), i for +x and -x (f = 1, 3), k for -z and +z (f = 4, 5).
+static int lift_normal_index(int f, int i, int j, int k) {
+  return f == 0 || f == 2 ? j : f == 1 || f == 3 ? i : k;
+}
+
+bool pack_lift_deltas(int Nq, const double *Lift, double *out) {
+  const int Nfp = Nq * Nq, Np = Nfp * Nq;
+  auto bits = [](double x) {
+    int64_t b;
+    std::memcpy(&b, &x, sizeof b);
+    return b;
+  };
+  // the groups (f, n): every value of face f's column with normal index n
+  std::vector<int64_t> lo(6 * Nq, INT64_MAX), hi(6 * Nq, INT64_MIN);
+  std::vector<int> neg(6 * Nq, 0), cnt(6 * Nq, 0);
+  for (int f = 0; f < 6; ++f)
+    for (int k = 0; k < Nq; ++k)
+      for (int j = 0; j < Nq; ++j)
+        for (int i = 0; i < Nq; ++i) {
+          const double x = Lift[i + Nq * j + Nfp * k + (size_t)Np * f];
+          const int g = f * Nq + lift_normal_index(f, i, j, k);
+          const int64_t b = bits(x);
+          lo[g] = std::min(lo[g], b);
+          hi[g] = std::max(hi[g], b);
+          neg[g] += std::signbit(x) ? 1 : 0;
+          ++cnt[g];
+          if (!std::isfinite(x)) return false;
+        }
+  std::vector<int64_t> base(6 * Nq);
+  for (int g = 0; g < 6 * Nq; ++g) {
+    if (neg[g] != 0 && neg[g] != cnt[g]) return false;  // one sign per group
+    if (hi[g] - lo[g] > 255) return false;
+    base[g] = lo[g];
+    std::memcpy(&out[g], &base[g], sizeof(double));
+  }
+  auto *bytes = reinterpret_cast<uint8_t *>(out + 6 * Nq);
+  std::memset(bytes, 0, (size_t)Np * 8);
+  for (int pl = 0; pl < Nfp; ++pl)
+    for (int k = 0; k < Nq; ++k)
+      for (int f = 0; f < 6; ++f) {
+        const int i = pl % Nq, j = pl / Nq;
+        const int64_t d =
+            bits(Lift[pl + Nfp * k + (size_t)Np * f]) - base[f * Nq + lift_normal_index(f, i, j, k)];
+        bytes[((size_t)k * Nfp + pl) * 8 + f] = (uint8_t)d;
+      }
+  return true;
+}
+
 }  // namespace dgk
 
 using namespace dgk;

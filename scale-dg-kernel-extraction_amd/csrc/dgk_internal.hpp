@@ -3,6 +3,8 @@
 
 #include "../../include/dgk.h"
 
+#include <cstddef>
+
 namespace dgk {
 
 // Face f (0-based: 0 -y, 1 +x, 2 +y, 3 -x, 4 -z, 5 +z): normal axis, normal
@@ -32,6 +34,20 @@ void clear_error();
 void split_tables(int Nq, const double *drow, double *d32);
 // Legendre-Gauss-Lobatto nodes x[Nq] on [-1,1], ascending.
 void lgl_nodes(int Nq, double *x);
+
+// The dense lift Lift(i,j,k,f) (the reference's (Np,6) layout) packed for
+// the pencil kernels without loss: each value is the double whose bit
+// pattern is its group's base plus an unsigned byte, the group being face f's
+// values with one normal index n (j for faces 0 and 2, i for 1 and 3, k for
+// 4 and 5), so that a lane reads 8 bytes per node instead of 6 doubles.
+// out (lift_delta_doubles(Nq) doubles): the 6 Nq bases (as doubles, group
+// f Nq + n: the group's smallest bit pattern), then one 8-byte word per
+// node, word k Nq^2 + pl for node k of line pl = i + Nq j, its byte f the
+// delta of Lift(i,j,k,f) (bytes 6, 7 zero).  False (out unspecified) when a
+// group mixes signs, holds a non-finite value or spans more than 255 units
+// in the last place.
+constexpr size_t lift_delta_doubles(int Nq) { return 6 * (size_t)Nq + (size_t)Nq * Nq * Nq; }
+bool pack_lift_deltas(int Nq, const double *Lift, double *out);
 
 // The device reductions of dgk_diagnostics over the n = Np Ne owned values
 // of q (elements of Nq^3 nodes; uniform Escale es, or the per-node escale

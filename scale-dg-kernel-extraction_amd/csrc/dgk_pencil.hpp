@@ -567,7 +567,7 @@ __device__ __forceinline__ void pencil_compute(PT &P, const double *__restrict__
                                                const double *__restrict__ lfac, double *blk,
                                                int pl, long long e, long long eb, bool act,
                                                PencilLoads<NQ> &L) {
-  // Lift: the pair-packed copy described at the lift loads below
+  // Lift: the delta-packed copy described at the lift loads below
   using S = PencilShape<NQ>;
   constexpr int NP = S::NP, NFP = S::NFP, PAD = S::PAD, FPAD = S::FPAD;
   // QG (lean only): the x- and y-lines of q are read from memory, not LDS
@@ -758,6 +758,24 @@ __device__ __forceinline__ void pencil_compute(PT &P, const double *__restrict__
   // ---- final z phase: lift, combine, RK ----------------------------------
   if (!valid) return;
   const int i = la, j = lb;
+  // Dense lift, delta-packed (pack_lift_deltas): Lift(i,j,k,f) is the double
+  // whose bits are those of the base of (f, normal index) plus the unsigned
+  // byte f of this line's 8-byte word for node k, word [k NFP + pl] (lanes
+  // contiguous).  The bases of the four side faces are fixed along the line;
+  // those of -z and +z follow k and are the same for every lane.
+  long long b0 = 0, b1 = 0, b2 = 0, b3 = 0;
+  const long long *LB = reinterpret_cast<const long long *>(Lift);
+  const uint2 *LW = reinterpret_cast<const uint2 *>(Lift + 6 * NQ);
+  if constexpr (!FACT_LIFT) {
+    b0 = LB[0 * NQ + j];
+    b1 = LB[1 * NQ + i];
+    b2 = LB[2 * NQ + j];
+    b3 = LB[3 * NQ + i];
+  }
+  auto delta = [](const uint2 &wd, int f) -> long long {  // byte f of a word
+    const unsigned h = f < 4 ? wd.x : wd.y;
+    return (long long)((h >> (8 * (f % 4))) & 0xffu);
+  };
   if constexpr (LEAN) {
     if constexpr (!kEarly) load_q0();
 #pragma unroll
@@ -782,17 +800,13 @@ __device__ __forceinline__ void pencil_compute(PT &P, const double *__restrict__
       L[4] = lfac[k];
       L[5] = lfac[NQ + k];
     } else {
-      // Dense lift coefficients Lift(i,j,k,f) of this lane's z-line, packed
-      // by the host as 16-byte pairs that the lanes of a wavefront read
-      // contiguously: Lift[(m*NFP + line)*2 + {0,1}] holds values 2m, 2m+1
-      // of the line, value v = 6k + f.
-      const double2 *LP = reinterpret_cast<const double2 *>(Lift);
-#pragma unroll
-      for (int m = 0; m < 3; ++m) {  // values 6k..6k+5 = pairs 3k..3k+2
-        const double2 pr = LP[(3 * k + m) * NFP + pl];
-        L[2 * m] = pr.x;
-        L[2 * m + 1] = pr.y;
-      }
+      const uint2 wd = LW[k * NFP + pl];
+      L[0] = __longlong_as_double(b0 + delta(wd, 0));
+      L[1] = __longlong_as_double(b1 + delta(wd, 1));
+      L[2] = __longlong_as_double(b2 + delta(wd, 2));
+      L[3] = __longlong_as_double(b3 + delta(wd, 3));
+      L[4] = __longlong_as_double(LB[4 * NQ + k] + delta(wd, 4));
+      L[5] = __longlong_as_double(LB[5 * NQ + k] + delta(wd, 5));
     }
     // tensorprod_Lift_hexahedral order (mod_dg_optr_kernel.f90:277-283).
     // The mixed form lifts in FP64 too (fused): a lift term has no

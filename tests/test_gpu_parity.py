@@ -213,6 +213,58 @@ def test_tendency_bit_exact(gpu, golden, case, tname):
     assert CS.digest(d) == want
 
 
+def _lift_cases(L, p):
+    """Lifts the pencil kernels' packed form must reproduce bit for bit or
+    refuse: (name, Lift, packs).  The packed form holds each value as its
+    group's smallest bit pattern plus a byte, a group being one face's
+    values with one normal index, e.g. face -z (f=4) at depth k: L[:,:,k,4]."""
+    Nq = p + 1
+    near = L.copy()  # moved by up to 200 units in the last place, within groups
+    rng = np.random.default_rng(5)
+    bits = near.view(np.int64)
+    bits += rng.integers(0, 200, size=bits.shape)  # away from zero, either sign
+    wide = L.copy()  # one value of group (f=4, k=1) 300 units away
+    wide.view(np.int64)[Nq // 2, 0, 1, 4] += 300
+    sign = L.copy()  # one value of group (f=1, i=Nq-1) with the other sign
+    sign[Nq - 1, 1, 0, 1] = -sign[Nq - 1, 1, 0, 1]
+    return [("near", near, True), ("wide", wide, False), ("sign", sign, False)]
+
+
+@pytest.mark.parametrize("p", [3, 7])
+def test_lift_packing_is_lossless_or_refused(gpu, p):
+    """The pencil kernels read the dense Lift as byte offsets from group
+    bases.  A Lift whose groups span up to 255 units in the last place runs
+    on them and gives the oracle's bits with that Lift; one that does not
+    pack (a group 300 units wide, a group of mixed signs) runs on the node
+    kernel, bit-exact too, and the forms that need the pencil kernel ask for
+    the factored lift."""
+    ne = (3, 2, 2)
+    _, D, L = dgk.read_operator(p)
+    q, u, v, w = CS.tend_inputs(ne, p)
+    om = O.Mesh(*ne, p)
+    F = [full_field(a, om.NeA) for a in (q, u, v, w)]
+    for f in F:
+        om.update_halo(f)
+    for name, Lx, packs in _lift_cases(L, p):
+        say(f"[lift pack] p={p} {name}")
+        om.Lift = np.asfortranarray(Lx)
+        want = om.cal_tend(*F, O.OPT1)[:, :om.Ne]
+        s = dgk.Solver.structured(*ne, p, D, Lx, typeid=dgk.TYPE_OPT1)
+        assert s.kernel_kind == ("pencil" if packs else "node"), name
+        s.set_fields(q, u, v, w)
+        assert np.array_equal(s.cal_tend(), want), name
+        s.close()
+        if packs:
+            continue
+        with pytest.raises(dgk.DGKError, match="FACTORED_LIFT"):
+            dgk.Solver.structured(*ne, p, D, Lx, typeid=dgk.TYPE_OPT1,
+                                  flags=dgk.FLAG_CONST_VELOCITY)
+        s = dgk.Solver.structured(*ne, p, D, Lx, typeid=dgk.TYPE_OPT1,
+                                  flags=dgk.FLAG_CONST_VELOCITY | dgk.FLAG_FACTORED_LIFT)
+        assert s.kernel_kind == "pencil"
+        s.close()
+
+
 # ---------------------------------------------------------------------------
 # trajectories (main.f90 step loop)
 
