@@ -27,6 +27,12 @@ and the components either side of the stage kernel on the workload's mesh
 --copy-calibration first copies 1 GiB device to device (hipMemcpy's copy
 kernel) three times, so that a rocprofv3 --pmc run of this script also
 records what FETCH_SIZE/WRITE_SIZE read for a copy of known size.
+
+Unless --no-traffic or --no-extra, the HBM traffic per launch of every stage
+kernel reported (roofline.traffic for the headline) is measured in the same
+run: two child processes of this script under rocprofv3 --pmc (FETCH_SIZE,
+then WRITE_SIZE) run one SSP-RK3 step of each variant with its tuning
+variant, after three 1 GiB device copies that calibrate each counter.
 """
 from __future__ import annotations
 
@@ -52,7 +58,6 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_DOF = (40 + 48 + 48) / 3
 # uniform velocity held as three scalars: 16 B at stage 1, 24 B after
 BYTES_PER_DOF_CV = (16 + 24 + 24) / 3
-PROFILES = os.path.join(REPO, "profiles")
 # solver variants: flags of dgk_create_structured
 VARIANTS = {
     "factored": dgk.FLAG_FACTORED_LIFT,                # FP64, lift by the 1D vectors
@@ -67,6 +72,7 @@ VARIANTS = {
     "fma_factored_constvel": dgk.FLAG_FMA | dgk.FLAG_FACTORED_LIFT | dgk.FLAG_CONST_VELOCITY,
     # FP32 contractions of each line's remainder after its cubic part
     "mixed": dgk.FLAG_MIXED,
+    "mixed_constvel": dgk.FLAG_MIXED | dgk.FLAG_CONST_VELOCITY,
     # p=15, uniform velocity: contractions on the matrix cores, FP64 MFMA and
     # FP32 MFMA with hi/lo-split operator and operand
     "mfma_constvel": dgk.FLAG_MFMA | dgk.FLAG_CONST_VELOCITY,
@@ -159,13 +165,104 @@ def cpu_baseline(threads, nstep, nrep, spec, q):
     return out
 
 
-def traffic_from_profiles(workload, variant):
-    path = os.path.join(PROFILES, f"hbm_traffic_{workload}_{variant}.json")
-    if not os.path.exists(path):
-        return None
-    with open(path) as f:
-        t = json.load(f)
-    return t.get("bytes_per_launch")
+STAGE_KERNELS = ("pencil_kernel", "pencil_group_kernel", "stage_kernel", "mfma16_kernel")
+
+
+def traffic_probe(plan):
+    """Child mode of measure_traffic, run under rocprofv3 --pmc: three 1 GiB
+    device-to-device copies (the counters' calibration), then for each
+    (workload, variant, tuning variant) of the plan a solver set up as in
+    the timed runs and one SSP-RK3 step of stage launches (three dispatches
+    of its stage kernel)."""
+    import torch
+    a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    del a, b
+    fields = {}
+    for workload, name, tv in plan:
+        spec = ic.WORKLOADS[workload]
+        ne, p, vel = spec["ne"], spec["p"], spec["vel"]
+        x1D, D, L = dgk.read_operator(p)
+        if workload not in fields:
+            fields.clear()
+            fields[workload] = _fields(spec, x1D)
+        q = fields[workload]
+        s = dgk.Solver.structured(*ne, p, D, L, flags=VARIANTS[name])
+        s.set_fields(q, *(np.full_like(q, c) for c in vel))
+        if tv is not None:
+            s.set_variant(tv)
+        s.synchronize()
+        s.time_stages(ic.cfl_dt(p, vel, ne), 3)
+        s.close()
+        log(f"traffic probe: {workload} {name} (tuning variant {tv})")
+
+
+def measure_traffic(plan, timeout=300):
+    """HBM bytes per stage launch of each (workload, variant, tuning variant)
+    in plan, measured in this run: two child processes of this script
+    (traffic_probe) under rocprofv3 --pmc, one for FETCH_SIZE and one for
+    WRITE_SIZE (KiB per dispatch), each scaled by what the counter reads for
+    a 1 GiB copy in the same process.  Returns ({(workload, variant):
+    record}, None) or (None, reason)."""
+    import csv
+    import shutil
+    import signal
+    import subprocess
+    import tempfile
+    prof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not os.path.exists(prof):
+        return None, "rocprofv3 not found"
+    per = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="dgk_pmc_", dir="/tmp")
+        cmd = [prof, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "p", "--",
+               sys.executable, os.path.abspath(__file__), "--traffic-probe", json.dumps(plan)]
+        log(f"traffic: {counter} pass ({len(plan)} variants) under rocprofv3")
+        proc = subprocess.Popen(cmd, cwd="/tmp", env=dict(os.environ, TMPDIR="/tmp"),
+                                stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
+                                start_new_session=True)
+        try:
+            rc = proc.wait(timeout=timeout)
+        except subprocess.TimeoutExpired:
+            os.killpg(proc.pid, signal.SIGKILL)
+            proc.wait()
+            shutil.rmtree(d, ignore_errors=True)
+            return None, f"{counter} pass exceeded {timeout} s"
+        rows = []
+        for root, _, files in os.walk(d):
+            for fn in files:
+                if fn.endswith("counter_collection.csv"):
+                    with open(os.path.join(root, fn)) as f:
+                        rows += [r for r in csv.DictReader(f) if r["Counter_Name"] == counter]
+        shutil.rmtree(d, ignore_errors=True)
+        rows.sort(key=lambda r: int(r["Dispatch_Id"]))
+        # the calibration copies are the probe's first copy dispatches (the
+        # uploads that follow use the same copy kernel)
+        copy = [float(r["Counter_Value"]) for r in rows if "copyBuffer" in r["Kernel_Name"]][:3]
+        stage = [float(r["Counter_Value"]) for r in rows
+                 if any(k + "<" in r["Kernel_Name"] or k + "(" in r["Kernel_Name"]
+                        for k in STAGE_KERNELS)]
+        # rocprofv3 may end with a fault of its own after complete results
+        if len(copy) < 3 or len(stage) != 3 * len(plan):
+            return None, (f"{counter} pass: rc {rc}, {len(copy)} copy and {len(stage)} stage "
+                          f"dispatches (expected 3 and {3 * len(plan)})")
+        if max(copy) > 1.05 * min(copy) or min(copy) < (1 << 20) / 4:
+            return None, f"{counter} pass: calibration copies read {copy} KiB"
+        scale = (1 << 30) / (1024.0 * float(np.median(copy)))
+        per[counter] = (scale, [1024.0 * v * scale for v in stage])
+    out = {}
+    (fs, fetch), (ws, write) = per["FETCH_SIZE"], per["WRITE_SIZE"]
+    for i, (workload, name, tv) in enumerate(plan):
+        f3, w3 = fetch[3 * i:3 * i + 3], write[3 * i:3 * i + 3]
+        out[(workload, name)] = {
+            "bytes_per_launch": round(sum(f3 + w3) / 3),
+            "read_bytes_per_stage": [round(x) for x in f3],
+            "write_bytes_per_stage": [round(x) for x in w3],
+            "tuning_variant": tv, "fetch_scale": round(fs, 4), "write_scale": round(ws, 4)}
+    return out, None
 
 
 def _fields(spec, x1D):
@@ -215,7 +312,7 @@ def mixed_workload(workload, x1D, D, L, spec, nstage=30):
     uvw = [np.full_like(q, c) for c in vel]
     res = {}
     for name, flags in (("per_node_velocity", VARIANTS["mixed"]),
-                        ("uniform_velocity", VARIANTS["mixed"] | dgk.FLAG_CONST_VELOCITY)):
+                        ("uniform_velocity", VARIANTS["mixed_constvel"])):
         s = dgk.Solver.structured(*ne, p, D, L, flags=flags)
         s.set_fields(q, *uvw)
         var, _ = s.autotune(dt, nrep=6)
@@ -598,6 +695,9 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--cpu-reps", type=int, default=3)
     ap.add_argument("--copy-calibration", action="store_true")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the rocprofv3 passes that measure HBM traffic per launch")
+    ap.add_argument("--traffic-probe", default=None, help=argparse.SUPPRESS)
     ap.add_argument("--no-error", action="store_true",
                     help="skip the nstep error run against the bit-exact variant")
     ap.add_argument("--no-extra", action="store_true",
@@ -613,6 +713,9 @@ def main():
               "sharding of the mesh is a design, not a measured path", file=sys.stderr)
         sys.exit(2)
 
+    if args.traffic_probe:
+        traffic_probe([tuple(x) for x in json.loads(args.traffic_probe)])
+        return
     if args.ensemble_only:
         print(json.dumps({"workload": "c4_ensemble_64", **ensemble_workload()}), flush=True)
         return
@@ -682,11 +785,13 @@ def main():
     finite = bool(np.isfinite(qf).all())
     # the bit-identical variant's stage kernel, timed on the same state
     ms_exact = None
+    tuning = {args.variant: s.variant}
     if args.variant != "exact":
         se = dgk.Solver.structured(*ne, p, D, L, flags=VARIANTS["exact"])
         se.set_fields(qf, *(np.full_like(qf, c) for c in vel))
         se.run(dt, 1)
         ms_exact = se.time_stages(dt, nstage)
+        tuning["exact"] = se.variant
         se.close()
     # the sine-mode field has (near) zero mean: drift is measured against the
     # integral of |q|
@@ -727,6 +832,45 @@ def main():
         # the rows either side of the stage kernel, on the same mesh
         extra["side_rows_" + args.workload] = side_rows(spec, qf)
 
+    # HBM traffic per launch of every stage kernel reported above, measured
+    # in this run (rocprofv3 passes in child processes)
+    traffic, traffic_note = None, "not measured (--no-traffic or --no-extra)"
+    if not (args.no_traffic or args.no_extra):
+        rows = [(args.workload, name, tv, None) for name, tv in tuning.items()]
+        mx = extra["c3_64cube_p7_mixed"]
+        rows += [("c3_64cube_p7", "mixed", mx["per_node_velocity"]["variant"],
+                  mx["per_node_velocity"]),
+                 ("c3_64cube_p7", "mixed_constvel", mx["uniform_velocity"]["variant"],
+                  mx["uniform_velocity"])]
+        for key, workload in (("c3_64cube_p7_other_variants", "c3_64cube_p7"),
+                              ("c5_16cube_p15", "c5_16cube_p15"), ("c2_32cube_p7", "c2_32cube_p7"),
+                              ("c1_8cube_p3", "c1_8cube_p3")):
+            for name in VARIANTS:
+                if isinstance(extra[key].get(name), dict):
+                    rows.append((workload, name, extra[key][name]["variant"], extra[key][name]))
+        traffic, traffic_note = measure_traffic([r[:3] for r in rows])
+        if traffic is None:
+            log(f"traffic not measured: {traffic_note}")
+        else:
+            traffic_note = ("rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE in child processes of this "
+                            "run, one SSP-RK3 step per variant, each counter scaled by its "
+                            "reading for a 1 GiB device copy")
+            for workload, name, _, rec in rows:
+                tr = traffic[(workload, name)]
+                sp = ic.WORKLOADS[workload]
+                tr["algorithmic_bytes_per_launch"] = round(
+                    (BYTES_PER_DOF_CV if VARIANTS[name] & dgk.FLAG_CONST_VELOCITY
+                     else BYTES_PER_DOF) * sp["ne"][0] * sp["ne"][1] * sp["ne"][2] *
+                    (sp["p"] + 1) ** 3)
+                tr["vs_algorithmic"] = round(tr["bytes_per_launch"] /
+                                             tr["algorithmic_bytes_per_launch"], 4)
+                if rec is not None:
+                    rec["traffic_bytes_per_launch"] = tr["bytes_per_launch"]
+                    rec["traffic_vs_algorithmic"] = tr["vs_algorithmic"]
+            log("traffic measured for " + ", ".join(f"{w}/{n}" for w, n, _, _ in rows))
+    head_traffic = traffic.get((args.workload, args.variant)) if traffic else None
+    exact_traffic = traffic.get((args.workload, "exact")) if traffic else None
+
     cpu = None
     if not args.no_cpu_baseline:
         # the CPUs this job may use: the OpenMP share the host gives it (16
@@ -756,9 +900,12 @@ def main():
                    "DOF": dof, "dof_updates_per_step": 3 * dof},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic_from_profiles(args.workload, args.variant),
+                     "traffic": head_traffic["bytes_per_launch"] if head_traffic else None,
                      "kernel": kernel_name, "kernel_us": round(ms_stage * 1e3, 2),
-                     "algorithmic_bytes_per_launch": bytes_launch},
+                     "algorithmic_bytes_per_launch": bytes_launch,
+                     "traffic_vs_algorithmic": head_traffic["vs_algorithmic"] if head_traffic
+                     else None,
+                     "traffic_detail": head_traffic, "traffic_source": traffic_note},
         "cpu_baseline": cpu,
         "checks": {"finite": finite, "mass_drift_rel": drift},
         "error_vs_fp64_oracle": err,
@@ -767,7 +914,10 @@ def main():
             "kernel_us": round(ms_exact * 1e3, 2),
             "gdof_updates_per_s": round(dof / (ms_exact * 1e-3) / 1e9, 2),
             # the bit-identical variant streams per-node velocity
-            "hbm_frac": round(BYTES_PER_DOF * dof / (ms_exact * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+            "hbm_frac": round(BYTES_PER_DOF * dof / (ms_exact * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "traffic_bytes_per_launch": exact_traffic["bytes_per_launch"] if exact_traffic
+            else None,
+            "traffic_vs_algorithmic": exact_traffic["vs_algorithmic"] if exact_traffic else None},
     }
     print(json.dumps(out), flush=True)
 
