@@ -73,8 +73,9 @@ VARIANTS = {
     # FP32 contractions of each line's remainder after its cubic part
     "mixed": dgk.FLAG_MIXED,
     "mixed_constvel": dgk.FLAG_MIXED | dgk.FLAG_CONST_VELOCITY,
-    # p=15, uniform velocity: contractions on the matrix cores, FP64 MFMA and
-    # FP32 MFMA with hi/lo-split operator and operand
+    # p=15, uniform velocity: contractions on the matrix cores, FP64 MFMA, and
+    # FP32 MFMA on each line's remainder after an FP64 degree-7 fit (the fit
+    # and its derivative as FP64 MFMA products)
     "mfma_constvel": dgk.FLAG_MFMA | dgk.FLAG_CONST_VELOCITY,
     "mfma_mixed_constvel": dgk.FLAG_MFMA | dgk.FLAG_MIXED | dgk.FLAG_CONST_VELOCITY,
 }
@@ -392,14 +393,19 @@ def single_workload(workload, variants, nstage=30, errors=False):
              "tflops": round(flops_per_dof(p) * dofs / 1e12, 2),
              "fp64_peak_frac": round(flops_per_dof(p) * dofs / 1e12 / MATRIX_PEAK_TFLOPS["fp64"], 4)}
         if VARIANTS[name] & dgk.FLAG_MFMA:
-            # flops the matrix cores execute: the three sweeps (2 Nq per DOF
-            # each), three products each in the hi/lo-split FP32 form
+            # flops the matrix cores execute per DOF-update: the three sweeps,
+            # 2 Nq each (FP64; in the mixed form FP32 on the remainders), and
+            # in the mixed form the FP64 fit and correction products, 2 x 8
+            # nodes x 2 per direction; the fraction is of the matrix cores'
+            # time at their dense peaks (78.6 FP64, 157.3 FP32 TFLOP/s)
             mixed = bool(VARIANTS[name] & dgk.FLAG_MIXED)
-            mf = 6 * (p + 1) * (3 if mixed else 1)
-            peak = MATRIX_PEAK_TFLOPS["fp32" if mixed else "fp64"]
-            r.update(matrix_tflops_executed=round(mf * dofs / 1e12, 2),
-                     matrix_peak_tflops=peak,
-                     matrix_peak_frac=round(mf * dofs / 1e12 / peak, 4))
+            f32 = 6 * (p + 1) if mixed else 0
+            f64 = 3 * 2 * 8 * 2 if mixed else 6 * (p + 1)
+            r.update(matrix_tflops_executed_fp64=round(f64 * dofs / 1e12, 2),
+                     matrix_tflops_executed_fp32=round(f32 * dofs / 1e12, 2),
+                     matrix_peak_tflops=MATRIX_PEAK_TFLOPS,
+                     matrix_peak_frac=round(f64 * dofs / 1e12 / MATRIX_PEAK_TFLOPS["fp64"] +
+                                            f32 * dofs / 1e12 / MATRIX_PEAK_TFLOPS["fp32"], 4))
         if ref is not None:
             s.set_fields(q, *uvw)
             s.run(dt, spec["nstep"])
@@ -588,6 +594,34 @@ class _Hip:
         self.h.hipEventDestroy(e0)
         self.h.hipEventDestroy(e1)
         return ms.value / nrep * 1e3
+
+
+def streaming_reference(gib=4, nrep=20):
+    """What a plain streaming kernel moves on this GPU, for scale beside the
+    stage kernel: torch.addcmul (three reads, one write of `gib` GiB each)
+    and a copy, timed with events on torch's stream."""
+    import torch
+    n = (gib << 30) // 8
+    a, b, c = (torch.ones(n, dtype=torch.float64, device="cuda") for _ in range(3))
+    o = torch.empty_like(a)
+    out = {}
+    for name, fn, nbytes in (("addcmul_3r1w", lambda: torch.addcmul(c, a, b, out=o), 4),
+                             ("copy_1r1w", lambda: o.copy_(a), 2)):
+        for _ in range(3):
+            fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(nrep):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / nrep
+        gbs = nbytes * (gib << 30) / (ms * 1e-3) / 1e9
+        out[name] = {"GBps": round(gbs, 1), "frac_of_peak": round(gbs / HBM_PEAK_GBS, 4)}
+    del a, b, c, o
+    torch.cuda.empty_cache()
+    out["arrays"] = f"{gib} GiB FP64 each, torch kernels, HIP events over {nrep} launches"
+    return out
 
 
 def side_rows(spec, q, nrep=5):
@@ -871,6 +905,8 @@ def main():
     head_traffic = traffic.get((args.workload, args.variant)) if traffic else None
     exact_traffic = traffic.get((args.workload, "exact")) if traffic else None
 
+    stream_ref = None if args.no_extra else streaming_reference()
+
     cpu = None
     if not args.no_cpu_baseline:
         # the CPUs this job may use: the OpenMP share the host gives it (16
@@ -905,7 +941,8 @@ def main():
                      "algorithmic_bytes_per_launch": bytes_launch,
                      "traffic_vs_algorithmic": head_traffic["vs_algorithmic"] if head_traffic
                      else None,
-                     "traffic_detail": head_traffic, "traffic_source": traffic_note},
+                     "traffic_detail": head_traffic, "traffic_source": traffic_note,
+                     "streaming_reference": stream_ref},
         "cpu_baseline": cpu,
         "checks": {"finite": finite, "mass_drift_rel": drift},
         "error_vs_fp64_oracle": err,

@@ -804,7 +804,7 @@ int finish_create(dgk_solver *s, const double *D1D, const double *Lift) {
   if ((rc = dalloc(&s->D, nq2)) || (rc = dalloc(&s->Lift, nlift)) ||
       (rc = dalloc(&s->lfac, 2 * (size_t)Nq)) || (rc = dalloc(&s->Drow, nq2)) ||
       (rc = dalloc(&s->LiftP, lift_delta_doubles(Nq))) ||
-      (rc = dalloc(&s->Drow32, nq2 + 3 * Nq)))
+      (rc = dalloc(&s->Drow32, nq2 + 3 * Nq + (Nq == 16 ? kMfmaFitDoubles : 0))))
     return rc;
   // the delta-packed Lift of the pencil kernels (without it, an exact lift
   // takes the node kernel)
@@ -830,12 +830,15 @@ int finish_create(dgk_solver *s, const double *D1D, const double *Lift) {
   HIPOK(hipMemcpy(s->lfac, lf.data(), lf.size() * sizeof(double), hipMemcpyHostToDevice));
   s->ops.lfac = s->lfac;
   // FP32 split copies (x = hi + lo, hi = fl32(x), lo = fl32(x - hi)) and the
-  // node tables of the line fit
+  // node tables of the line fit; at Nq = 16 the matrix-core form's fit
+  // tables after them
   {
-    std::vector<double> d32(nq2 + 3 * Nq);
+    std::vector<double> d32(nq2 + 3 * Nq + (Nq == 16 ? kMfmaFitDoubles : 0));
     split_tables(Nq, dr.data(), d32.data());
+    if (Nq == 16) mfma_fit_tables(dr.data(), d32.data() + nq2 + 3 * Nq);
     HIPOK(hipMemcpy(s->Drow32, d32.data(), d32.size() * sizeof(double), hipMemcpyHostToDevice));
     s->ops.Drow32 = s->Drow32;
+    s->ops.mfit = Nq == 16 ? s->Drow32 + nq2 + 3 * Nq : nullptr;
   }
   if ((rc = alloc_fields(s))) return rc;
   if ((rc = configure_launch(s))) return rc;

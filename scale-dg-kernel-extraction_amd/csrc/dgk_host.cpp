@@ -201,6 +201,28 @@ bool pack_lift_deltas(int Nq, const double *Lift, double *out) {
   return true;
 }
 
+
+void mfma_fit_tables(const double *drow, double *out) {
+  constexpr int Nq = 16;
+  double x[Nq];
+  lgl_nodes(Nq, x);
+  double *lb = out, *gd = out + 128;
+  for (int n = 0; n < Nq; ++n)
+    for (int f = 0; f < 8; ++f) {
+      double v = 1.0;
+      const double xf = x[kMfmaFitNodes[f]];
+      for (int h = 0; h < 8; ++h)
+        if (h != f) v *= (x[n] - x[kMfmaFitNodes[h]]) / (xf - x[kMfmaFitNodes[h]]);
+      lb[8 * n + f] = v;
+    }
+  for (int n = 0; n < Nq; ++n)
+    for (int f = 0; f < 8; ++f) {
+      double s = 0.0;
+      for (int l = 0; l < Nq; ++l) s += drow[n * Nq + l] * lb[8 * l + f];
+      gd[8 * n + f] = s;
+    }
+}
+
 }  // namespace dgk
 
 using namespace dgk;

@@ -49,6 +49,14 @@ void lgl_nodes(int Nq, double *x);
 constexpr size_t lift_delta_doubles(int Nq) { return 6 * (size_t)Nq + (size_t)Nq * Nq * Nq; }
 bool pack_lift_deltas(int Nq, const double *Lift, double *out);
 
+// Line-fit tables of the matrix-core mixed form (Nq = 16): out[8 n + f] =
+// l_f(x_n), the Lagrange basis on the nodes f of kMfmaFitNodes at LGL node
+// n, and out[128 + 8 n + f] = sum_l D(n,l) l_f(x_l), from the row-major
+// operator drow.  kMfmaFitDoubles doubles.
+constexpr int kMfmaFitNodes[8] = {0, 2, 4, 6, 9, 11, 13, 15};  // fit_node in dgk_mfma.hip
+constexpr int kMfmaFitDoubles = 256;
+void mfma_fit_tables(const double *drow, double *out);
+
 // The device reductions of dgk_diagnostics over the n = Np Ne owned values
 // of q (elements of Nq^3 nodes; uniform Escale es, or the per-node escale
 // (Np,Ne,3) if not null), against the device field ref (or null): out[8] as

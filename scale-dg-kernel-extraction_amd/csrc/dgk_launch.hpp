@@ -15,6 +15,7 @@ struct HostOps {
   const double *lfac;
   const double *LiftP;   // pair-packed Lift for the pencil kernel
   const double *Drow32;  // mixed precision: split rows and line-fit tables (split_tables)
+  const double *mfit;    // Nq = 16, matrix-core mixed form: line-fit tables (mfma_fit_tables)
 };
 
 struct StageKernel;
@@ -41,9 +42,9 @@ StageKernel pick_pencil_group_low(int Nq, int order, bool fma, bool fact, bool c
 StageKernel pick_pencil_group_mid(int Nq, int order, bool fma, bool fact, bool cv, int epb);
 StageKernel pick_pencil_group_high(int Nq, int order, bool fma, bool fact, bool cv, int epb);
 // The matrix-core stage (dgk_mfma.hip) for Nq = 16 with uniform velocity:
-// FP64 MFMA contractions, or with `mixed` FP32 MFMA with hi/lo-split
-// operator and operand; `wide`: eight wavefronts per element instead of
-// four.  An empty descriptor for other Nq.
+// FP64 MFMA contractions, or with `mixed` FP32 MFMA contractions of each
+// line's remainder after an FP64 degree-7 fit; `wide`: eight wavefronts per
+// element instead of four.  An empty descriptor for other Nq.
 StageKernel pick_mfma(int Nq, bool mixed, bool wide);
 
 }  // namespace dgk
