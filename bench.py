@@ -747,6 +747,15 @@ def main():
               "sharding of the mesh is a design, not a measured path", file=sys.stderr)
         sys.exit(2)
 
+    # torch's wheel carries its own HIP and HSA runtimes beside the ones
+    # libdgk.so links: torch's must take the GPU first (as in the tests), or
+    # its later initialisation finds no device once libdgk's has it
+    import torch
+    if not torch.cuda.is_available():
+        print("bench.py needs a GPU (torch.cuda.is_available() is False)", file=sys.stderr)
+        sys.exit(2)
+    torch.cuda.init()
+
     if args.traffic_probe:
         traffic_probe([tuple(x) for x in json.loads(args.traffic_probe)])
         return
