@@ -61,24 +61,29 @@ extern "C" {
                                       results (pencil kernel only)          */
 
 #define DGK_FLAG_MIXED 0x20         /* directional contractions in FP32:
-                                      each line's linear part is taken out
-                                      and differentiated in FP64, the
-                                      remainder contracted with the operator
-                                      split into hi + lo FP32 parts (two FP32
-                                      sums); lift in FP32 with split vectors
-                                      and operands; face-flux differences,
-                                      residual and RK in FP64.  Implies the
-                                      factored lift; pencil kernels only
-                                      (compressed connectivity), no FMA
-                                      flag */
+                                      each line's cubic through its nodes
+                                      0, Nq/3, 2Nq/3, Nq-1 is differentiated
+                                      in FP64 and only the remainder is
+                                      rounded to FP32 and contracted with
+                                      the operator split into hi + lo FP32
+                                      parts (two FP32 sums); a line whose
+                                      remainder exceeds 1/16 of its largest
+                                      value is contracted in FP64.  Lift,
+                                      face-flux differences, residual and
+                                      RK in FP64.  Implies the factored
+                                      lift; pencil kernels only (compressed
+                                      connectivity), no FMA flag */
 
 #define DGK_FLAG_MFMA 0x40          /* directional contractions on the matrix
-                                      cores: FP64 v_mfma_f64_16x16x4_f64, or
-                                      with DGK_FLAG_MIXED FP32
-                                      v_mfma_f32_16x16x4_f32 with operator
-                                      and operand split hi + lo (three
-                                      products); face fluxes, lift, residual
-                                      and RK in FP64.  PolyOrder 15 with
+                                      cores: FP64 v_mfma_f64_16x16x4_f64,
+                                      or with DGK_FLAG_MIXED FP32
+                                      v_mfma_f32_16x16x4_f32 on each line's
+                                      remainder after its degree-7
+                                      interpolant through 8 of its nodes
+                                      (interpolant and its derivative as
+                                      FP64 MFMA products); face fluxes,
+                                      lift, residual and RK in FP64.
+                                      PolyOrder 15 with
                                       DGK_FLAG_CONST_VELOCITY on a compressed
                                       mesh; implies the factored lift; fused
                                       (not bit-identical) */

@@ -643,6 +643,23 @@ __device__ __forceinline__ void pencil_compute(PT &P, const double *__restrict__
 #pragma unroll
     for (int k = 0; k < NQ; ++k) q0[k] = need_q0 ? ld_stream(P.q0 + base + NFP * k) : 0.0;
   };
+  // Dense lift, delta-packed (pack_lift_deltas): Lift(i,j,k,f) is the double
+  // whose bits are those of the base of (f, normal index) plus the unsigned
+  // byte f of this line's 8-byte word for node k, word [k NFP + pl] (lanes
+  // contiguous).  The bases of the four side faces are fixed along the line;
+  // those of -z and +z follow k and are the same for every lane.  The words
+  // of the lane's Nq nodes are requested together ahead of the final phase:
+  // loaded inside its loop, between the stores, each node waited on its own
+  // cache round trip.
+  const long long *LB = reinterpret_cast<const long long *>(Lift);
+  const uint2 *LW = reinterpret_cast<const uint2 *>(Lift + 6 * NQ);
+  uint2 lw[FACT_LIFT ? 1 : NQ];
+  auto load_lw = [&] {
+    if constexpr (!FACT_LIFT) {
+#pragma unroll
+      for (int k = 0; k < NQ; ++k) lw[k] = LW[k * NFP + pl];
+    }
+  };
   if constexpr (kEarly) load_ux();
   if constexpr (!QG) {
     if (act) {
@@ -708,7 +725,10 @@ __device__ __forceinline__ void pencil_compute(PT &P, const double *__restrict__
         }
       }
     }
-    if constexpr (kEarly) load_q0();
+    if constexpr (kEarly) {
+      load_q0();
+      load_lw();
+    }
     __syncthreads();
   } else {
     // ---- y phase: line (i,k) = (la,lb) -------------------------------------
@@ -758,14 +778,8 @@ __device__ __forceinline__ void pencil_compute(PT &P, const double *__restrict__
   // ---- final z phase: lift, combine, RK ----------------------------------
   if (!valid) return;
   const int i = la, j = lb;
-  // Dense lift, delta-packed (pack_lift_deltas): Lift(i,j,k,f) is the double
-  // whose bits are those of the base of (f, normal index) plus the unsigned
-  // byte f of this line's 8-byte word for node k, word [k NFP + pl] (lanes
-  // contiguous).  The bases of the four side faces are fixed along the line;
-  // those of -z and +z follow k and are the same for every lane.
+  if constexpr (!kEarly) load_lw();
   long long b0 = 0, b1 = 0, b2 = 0, b3 = 0;
-  const long long *LB = reinterpret_cast<const long long *>(Lift);
-  const uint2 *LW = reinterpret_cast<const uint2 *>(Lift + 6 * NQ);
   if constexpr (!FACT_LIFT) {
     b0 = LB[0 * NQ + j];
     b1 = LB[1 * NQ + i];
@@ -782,6 +796,14 @@ __device__ __forceinline__ void pencil_compute(PT &P, const double *__restrict__
     for (int k = 0; k < NQ; ++k) q[k] = QG ? P.qin[base + NFP * k] : Bq[pz(k)];
     z_phase(q);
   }
+  // The loop has no branch: the tendency launch (dqdt) and the RK stages (q)
+  // differ in a uniform choice of store target and value.  With the mode
+  // tested inside it every node was a basic block of its own and waited on
+  // its own scalar loads (the -z/+z lift coefficients of its k) instead of
+  // having them requested together.
+  const bool tend = P.mode == kModeTend;
+  double *const dst = tend ? P.dqdt : P.qout;
+  const bool stage1 = P.mode == kModeStage1;
 #pragma unroll
   for (int k = 0; k < NQ; ++k) {
     const int s = pz(k);
@@ -800,7 +822,7 @@ __device__ __forceinline__ void pencil_compute(PT &P, const double *__restrict__
       L[4] = lfac[k];
       L[5] = lfac[NQ + k];
     } else {
-      const uint2 wd = LW[k * NFP + pl];
+      const uint2 wd = lw[k];
       L[0] = __longlong_as_double(b0 + delta(wd, 0));
       L[1] = __longlong_as_double(b1 + delta(wd, 1));
       L[2] = __longlong_as_double(b2 + delta(wd, 2));
@@ -832,16 +854,12 @@ __device__ __forceinline__ void pencil_compute(PT &P, const double *__restrict__
     sum = madd<FMA>(e3s, dz[k], sum);
     const double dq = -(sum + lift);
     const long long idx = base + NFP * k;
-    if (P.mode == kModeTend) {
-      st<NT>(P.dqdt + idx, dq);
-    } else {
-      const double qq = q[k];
-      const double qs = P.mode == kModeStage1 ? qq : q0[k];
-      double qn;
-      if constexpr (FMA) qn = __builtin_fma(P.a, qs, P.b * __builtin_fma(P.dt, dq, qq));
-      else qn = P.a * qs + P.b * (qq + P.dt * dq);
-      st<NT>(P.qout + idx, qn);
-    }
+    const double qq = q[k];
+    const double qs = stage1 ? qq : q0[k];
+    double qn;
+    if constexpr (FMA) qn = __builtin_fma(P.a, qs, P.b * __builtin_fma(P.dt, dq, qq));
+    else qn = P.a * qs + P.b * (qq + P.dt * dq);
+    st<NT>(dst + idx, tend ? dq : qn);
   }
 }
 
