@@ -171,11 +171,8 @@ __global__ void __launch_bounds__(256) diag_partial_kernel(const DiagArgs A, dou
   const int dn = (int)(stride % A.Np);
   long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   int nd = (int)(t % A.Np);
-#pragma unroll 4
-  for (; t < A.n; t += stride) {
-    const double x = A.q[t];
-    const double J =
-        ESC ? 1.0 / (A.escale[t] * A.escale[t + A.NpNe] * A.escale[t + 2 * A.NpNe]) : J0;
+  auto add = [&](double x, double r, double e0, double e1, double e2) {
+    const double J = ESC ? 1.0 / (e0 * e1 * e2) : J0;
     const double wj = J * wn[nd];
     nd += dn;
     if (nd >= A.Np) nd -= A.Np;
@@ -185,12 +182,34 @@ __global__ void __launch_bounds__(256) diag_partial_kernel(const DiagArgs A, dou
     v[3] += wj * fabs(x);
     v[4] += x * x;
     if constexpr (REF) {
-      const double r = A.r[t], d = x - r;
+      const double d = x - r;
       v[5] += d * d;
       v[6] += r * r;
       v[7] = nan_max(v[7], fabs(d));
     }
+  };
+  // The values of kU strides are requested before any is reduced (left to
+  // the compiler, each load waited before the next was issued, the LDS
+  // weight lookup sitting between them); the reduction order is the
+  // thread's stride order either way.
+  constexpr int kU = 4;
+  for (; t + (kU - 1) * stride < A.n; t += kU * stride) {
+    double x[kU], r[kU], e[3][kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const long long tu = t + u * stride;
+      x[u] = A.q[tu];
+      r[u] = REF ? A.r[tu] : 0.0;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) e[d][u] = ESC ? A.escale[tu + d * A.NpNe] : 1.0;
+    }
+    __builtin_amdgcn_sched_barrier(0);  // every load above is issued first
+#pragma unroll
+    for (int u = 0; u < kU; ++u) add(x[u], r[u], e[0][u], e[1][u], e[2][u]);
   }
+  for (; t < A.n; t += stride)
+    add(A.q[t], REF ? A.r[t] : 0.0, ESC ? A.escale[t] : 1.0, ESC ? A.escale[t + A.NpNe] : 1.0,
+        ESC ? A.escale[t + 2 * A.NpNe] : 1.0);
   diag_block_reduce(v, part + kDiag * blockIdx.x);
 }
 
