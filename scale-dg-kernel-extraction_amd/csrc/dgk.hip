@@ -1032,18 +1032,18 @@ int dgk_set_fields(dgk_solver *s, const double *q, const double *u, const double
     const double *vel[3] = {u, v, w};
     const size_t owned = (size_t)s->Np * s->Ne;
     for (int d = 0; d < 3; ++d) {
-      for (size_t n = 1; n < owned; ++n)
-        if (vel[d][n] != vel[d][0])
-          return fail(DGK_ERR_ARG, "DGK_FLAG_CONST_VELOCITY: velocity component %d is not "
-                                   "uniform (node %zu)", d + 1, n + 1);
+      size_t n = 0;
+      if (owned > 1 && find_not_equal(vel[d] + 1, owned - 1, vel[d][0], &n))
+        return fail(DGK_ERR_ARG, "DGK_FLAG_CONST_VELOCITY: velocity component %d is not "
+                                 "uniform (node %zu)", d + 1, n + 2);
       s->cvel[d] = vel[d][0];
     }
     drop_graphs(s);  // the velocity is a launch argument
   }
   s->cur = 0;
   s->c0 = 2;
-  HIPOK(hipMemcpyAsync(s->q[0], q, nb, hipMemcpyHostToDevice, s->stream));
   int rc;
+  if ((rc = copy_h2d(s->q[0], q, nb, s->stream))) return rc;
   if ((rc = launch_halo(s, s->q[0]))) return rc;
   double *vel_dev[3] = {s->u, s->v, s->w};
   const double *vel_host[3] = {u, v, w};
@@ -1057,7 +1057,7 @@ int dgk_set_fields(dgk_solver *s, const double *q, const double *u, const double
                          s->cvel[d]);
       HIPOK(hipGetLastError());
     } else {
-      HIPOK(hipMemcpyAsync(vel_dev[d], vel_host[d], nb, hipMemcpyHostToDevice, s->stream));
+      if ((rc = copy_h2d(vel_dev[d], vel_host[d], nb, s->stream))) return rc;
       if ((rc = launch_halo(s, vel_dev[d]))) return rc;
     }
   }
@@ -1086,10 +1086,7 @@ int dgk_get_field(dgk_solver *s, int which, double *out) {
   if (rc) return rc;
   if (!out) return fail(DGK_ERR_ARG, "null output");
   if (which == DGK_FIELD_Q && (rc = launch_halo(s, src))) return rc;
-  HIPOK(hipMemcpyAsync(out, src, (size_t)s->Np * s->NeA * sizeof(double),
-                       hipMemcpyDeviceToHost, s->stream));
-  HIPOK(hipStreamSynchronize(s->stream));
-  return DGK_OK;
+  return copy_d2h(out, src, (size_t)s->Np * s->NeA * sizeof(double), s->stream);
 }
 
 int dgk_cal_tend(dgk_solver *s) {
@@ -1345,19 +1342,15 @@ int dgk_advect3d_eq_cal_tend(double *dqdt, const double *q, const double *u, con
   const size_t nb = (size_t)Np * NeA * sizeof(double);
   s->cur = 0;
   rc = DGK_OK;
-  if (hipMemcpy(s->q[0], q, nb, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(s->u, u, nb, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(s->v, v, nb, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(s->w, w, nb, hipMemcpyHostToDevice) != hipSuccess)
-    rc = fail(DGK_ERR_HIP, "field upload failed");
+  double *dev[4] = {s->q[0], s->u, s->v, s->w};
+  const double *host[4] = {q, u, v, w};
+  for (int b = 0; b < 4 && !rc; ++b) rc = copy_h2d(dev[b], host[b], nb, s->stream);
   s->fields_set = true;
   if (!rc) rc = launch_stage(s, kModeTend, 0.0, 0.0, 0.0);
   if (!rc && hipStreamSynchronize(s->stream) != hipSuccess)
     rc = fail(DGK_ERR_HIP, "tendency kernel failed");
   // dqdt(:,1:Ne) only, as cal_dqdt writes (mod_advect3d_eq.f90:194)
-  if (!rc && hipMemcpy(dqdt, s->dqdt, (size_t)Np * Ne * sizeof(double), hipMemcpyDeviceToHost) !=
-                 hipSuccess)
-    rc = fail(DGK_ERR_HIP, "dqdt download failed");
+  if (!rc) rc = copy_d2h(dqdt, s->dqdt, (size_t)Np * Ne * sizeof(double), s->stream);
   dgk_destroy(s);
   return rc;
 }

@@ -637,9 +637,9 @@ int dgk_ensemble_set_fields(dgk_ensemble *s, int i, const double *q, const doubl
   if (s->flags & DGK_FLAG_CONST_VELOCITY) {
     const double *vel[3] = {u, v, w};
     for (int d = 0; d < 3; ++d) {
-      for (size_t t = 1; t < cnt; ++t)
-        if (vel[d][t] != vel[d][0])
-          return fail(DGK_ERR_ARG, "instance %d: velocity component %d is not uniform", i, d + 1);
+      size_t t = 0;
+      if (cnt > 1 && find_not_equal(vel[d] + 1, cnt - 1, vel[d][0], &t))
+        return fail(DGK_ERR_ARG, "instance %d: velocity component %d is not uniform", i, d + 1);
       I.cvel[d] = vel[d][0];
     }
     HIPOK(hipMemcpy(s->dinst + i, &I, sizeof(EnsInst), hipMemcpyHostToDevice));
@@ -648,8 +648,10 @@ int dgk_ensemble_set_fields(dgk_ensemble *s, int i, const double *q, const doubl
   }
   double *dst[4] = {s->q[s->cur] + I.off, s->u + I.off, s->v + I.off, s->w + I.off};
   const double *src[4] = {q, u, v, w};
-  for (int b = 0; b < 4; ++b)
-    HIPOK(hipMemcpy(dst[b], src[b], cnt * sizeof(double), hipMemcpyHostToDevice));
+  for (int b = 0; b < 4; ++b) {
+    const int rc = copy_h2d(dst[b], src[b], cnt * sizeof(double), s->stream);
+    if (rc) return rc;
+  }
   s->fields_set = true;
   return DGK_OK;
 }
@@ -658,9 +660,7 @@ int dgk_ensemble_get_field(dgk_ensemble *s, int i, double *q) {
   if (!s || i < 0 || i >= s->n || !q) return fail(DGK_ERR_ARG, "bad argument");
   HIPOK(hipStreamSynchronize(s->stream));
   const EnsInst &I = s->inst[i];
-  HIPOK(hipMemcpy(q, s->q[s->cur] + I.off, (size_t)I.ne * s->np[i] * sizeof(double),
-                  hipMemcpyDeviceToHost));
-  return DGK_OK;
+  return copy_d2h(q, s->q[s->cur] + I.off, (size_t)I.ne * s->np[i] * sizeof(double), s->stream);
 }
 
 int dgk_ensemble_set_dt(dgk_ensemble *s, const double *dt) {

@@ -67,4 +67,14 @@ int device_diagnostics(const double *q, const double *ref, long long n, int Nq,
                        const double es[3], const double *escale, const double *w1D,
                        double *scratch, void *stream, double *out);
 
+// Synchronous copies of `bytes` between host memory (pageable) and device
+// memory on `stream` (a hipStream_t; its earlier work is ordered before the
+// copy): large copies go through two pinned staging buffers, filled and
+// emptied by several host threads while the DMA of the other buffer runs.
+int copy_h2d(void *dst, const void *src, size_t bytes, void *stream);
+int copy_d2h(void *dst, const void *src, size_t bytes, void *stream);
+// 1 and *where = the first index i < n with a[i] != v (NaN never equals),
+// else 0 and *where = n; the scan is split between the same host threads.
+int find_not_equal(const double *a, size_t n, double v, size_t *where);
+
 }  // namespace dgk
