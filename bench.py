@@ -378,9 +378,15 @@ def single_workload(workload, variants, nstage=30, errors=False):
     res = {"NeX": ne[0], "PolyOrder": p, "flops_per_dof_update": flops_per_dof(p)}
     ref = None
     if errors:
+        # the bit-identical variant (the oracle's bits) after nstep: the
+        # reference of every variant's rel-L2, max error and mass drift
         ref = dgk.Solver.structured(*ne, p, D, L, flags=VARIANTS["exact"])
         ref.set_fields(q, *uvw)
+        r0 = ref.diagnostics(x1D)
         ref.run(dt, spec["nstep"])
+        r1 = ref.diagnostics(x1D)
+        ref_scale = max(abs(r1["min"]), abs(r1["max"]))
+        res["oracle_mass_drift_rel"] = abs(r1["mass"] - r0["mass"]) / r0["abs_mass"]
     for name in variants:
         s = dgk.Solver.structured(*ne, p, D, L, flags=VARIANTS[name])
         s.set_fields(q, *uvw)
@@ -408,8 +414,14 @@ def single_workload(workload, variants, nstage=30, errors=False):
                                             f32 * dofs / 1e12 / MATRIX_PEAK_TFLOPS["fp32"], 4))
         if ref is not None:
             s.set_fields(q, *uvw)
+            m0 = s.diagnostics(x1D)
             s.run(dt, spec["nstep"])
-            r["rel_l2_vs_exact_after_nstep"] = s.diagnostics(x1D, ref=ref)["rel_l2"]
+            d = s.diagnostics(x1D, ref=ref)
+            r["rel_l2_vs_exact_after_nstep"] = d["rel_l2"]
+            r["error_vs_fp64_oracle"] = {
+                "nstep": spec["nstep"], "rel_l2": d["rel_l2"], "max_abs": d["max_err"],
+                "max_rel": d["max_err"] / ref_scale,
+                "mass_drift_rel": abs(d["mass"] - m0["mass"]) / m0["abs_mass"]}
         res[name] = r
         s.close()
         log(f"{workload} {name}: {r['gdof_updates_per_s']} GDOF-updates/s")
@@ -466,26 +478,48 @@ def ensemble_workload(nstage=30):
                     "variant": "FP32 contractions of the lines' remainders after their cubic "
                                "part, FP64 elsewhere, uniform velocity"}
     ens.close()
-    # mixed against the bit-identical FP64 ensemble after the workload's
-    # nstep, per instance
+    # the measured form and the mixed form against the bit-identical FP64
+    # ensemble after the workload's nstep, per instance: rel-L2, max error and
+    # the drift of the LGL-quadrature mass of each instance
     nstep = ic.ENSEMBLE["nstep"]
-    a = build(dgk.FLAG_CONST_VELOCITY | dgk.FLAG_MIXED)
     b = build(dgk.FLAG_CONST_VELOCITY)
-    a.run(nstep)
     b.run(nstep)
-    errs, num, den = [], 0.0, 0.0
-    for i in range(len(inst)):
-        x, y = b.get_field(i), a.get_field(i)
-        d2, r2 = float(np.sum((y - x) ** 2)), float(np.sum(x * x))
-        errs.append((d2 / r2) ** 0.5)
-        num += d2
-        den += r2
-    a.close()
+    ref = [b.get_field(i) for i in range(len(inst))]
     b.close()
-    res["mixed"]["error_vs_fp64"] = {
-        "nstep": nstep, "max_instance_rel_l2": max(errs),
-        "median_instance_rel_l2": float(np.median(errs)), "ensemble_rel_l2": (num / den) ** 0.5,
-        "reference": "the exact FP64 ensemble (bit-identical per instance to the oracle)"}
+    quad = []
+    for sp in inst:
+        w3 = np.einsum("i,j,k->ijk", *(lgl_weights(dgk.read_operator(sp["p"])[0]),) * 3)
+        quad.append((w3.ravel(order="F"),
+                     (1.0 / sp["ne"][0]) * (1.0 / sp["ne"][1]) * (1.0 / sp["ne"][2]) / 8.0))
+
+    def errors_of(flags):
+        a = build(flags)
+        a.run(nstep)
+        errs, num, den, mx, mx_rel, drift, drift_ref = [], 0.0, 0.0, 0.0, 0.0, 0.0, 0.0
+        for i in range(len(inst)):
+            x, y = ref[i], a.get_field(i)
+            d2, r2 = float(np.sum((y - x) ** 2)), float(np.sum(x * x))
+            errs.append((d2 / r2) ** 0.5)
+            num += d2
+            den += r2
+            e = float(np.max(np.abs(y - x)))
+            mx = max(mx, e)
+            mx_rel = max(mx_rel, e / float(np.max(np.abs(x))))
+            w3, J = quad[i]
+            m0, a0 = mass(fields[i], w3, J)
+            drift = max(drift, abs(mass(y, w3, J)[0] - m0) / a0)
+            drift_ref = max(drift_ref, abs(mass(x, w3, J)[0] - m0) / a0)
+        a.close()
+        return {"nstep": nstep, "max_instance_rel_l2": max(errs),
+                "median_instance_rel_l2": float(np.median(errs)),
+                "ensemble_rel_l2": (num / den) ** 0.5, "max_abs": mx,
+                "max_instance_max_rel": mx_rel, "max_instance_mass_drift_rel": drift,
+                "oracle_max_instance_mass_drift_rel": drift_ref,
+                "reference": "the exact FP64 ensemble (bit-identical per instance to the oracle)"}
+
+    res["error_vs_fp64_oracle"] = errors_of(
+        dgk.FLAG_CONST_VELOCITY | dgk.FLAG_FACTORED_LIFT | dgk.FLAG_FMA)
+    res["mixed"]["error_vs_fp64"] = errors_of(dgk.FLAG_CONST_VELOCITY | dgk.FLAG_MIXED)
     log(f"ensemble: {res['gdof_updates_per_s_kernel']} GDOF-updates/s per stage "
         f"(mixed {res['mixed']['gdof_updates_per_s_kernel']})")
     return res
