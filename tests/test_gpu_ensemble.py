@@ -362,3 +362,36 @@ def test_ensemble_workload_properties(gpu):
     assert checked == 2
     ens.close()
     mix.close()
+
+
+def test_ensemble_non_unit_domains(gpu):
+    """Instances on box domains of other lengths than 1 (per-axis element
+    widths differ between instances and axes) give the oracle's bits."""
+    specs = [dict(p=3, ne=(4, 3, 5), dom=(2.0, 0.5, 1.5)),
+             dict(p=7, ne=(3, 2, 3), dom=(0.25, 3.0, 1.0)),
+             dict(p=11, ne=(2, 2, 3), dom=(1.0, 1.0, 4.0))]
+    g = ic.SplitMix64(4343)
+    inst = []
+    for s in specs:
+        r = g.uniform(3)
+        vel = tuple(float(2.0 * r[d] - 1.0) for d in range(3))
+        x1D, D, L = dgk.read_operator(s["p"])
+        m = dgk.mesh_setup(*s["ne"], s["p"], x1D=x1D, dom=s["dom"], tables=False)
+        q = np.asfortranarray(np.sin(2.0 * np.pi * m["pos_en"][:, :, 0] / s["dom"][0]) +
+                              0.5 * np.cos(2.0 * np.pi * m["pos_en"][:, :, 2] / s["dom"][2]))
+        inst.append(dict(s, vel=vel, q=q, dt=ic.cfl_dt(s["p"], vel, s["ne"], dom=s["dom"])))
+    nstep = 3
+    say("[ens] non-unit domains, 3 steps")
+    ens = _ensemble(inst)
+    ens.run(nstep)
+    for i, s in enumerate(inst):
+        m = O.Mesh(*s["ne"], s["p"], dom=s["dom"])
+        F = []
+        for a in (s["q"], *(np.full_like(s["q"], c) for c in s["vel"])):
+            f = m.field()
+            f[:, :m.Ne] = a
+            m.update_halo(f)
+            F.append(f)
+        m.run(*F, s["dt"], nstep, O.OPT1)
+        assert np.array_equal(ens.get_field(i), F[0][:, :m.Ne]), i
+    ens.close()
