@@ -1,0 +1,122 @@
+// Host code of libdgk.so (csrc/dgk_host.cpp: the selector, the operator
+// reader, the mesh tables, the lift packing and the mixed-precision tables)
+// driven under AddressSanitizer and UndefinedBehaviorSanitizer by
+// tests/test_host_sanitize.py.  Every buffer is a vector of exactly the size
+// the interface documents, so that a write or read past it is reported.
+// Exit status 0 when every check holds; the sanitizers abort otherwise.
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../scale-dg-kernel-extraction_amd/csrc/dgk_internal.hpp"
+
+static int g_fail = 0;
+#define CHECK(c)                                                     \
+  do {                                                               \
+    if (!(c)) {                                                      \
+      std::fprintf(stderr, "check failed at line %d: %s\n", __LINE__, #c); \
+      ++g_fail;                                                      \
+    }                                                                \
+  } while (0)
+
+static void selector() {
+  int id = 0;
+  CHECK(dgk_optr_kernel_setup("GENERAL", &id) == DGK_OK && id == DGK_TYPE_GENERAL);
+  CHECK(dgk_optr_kernel_setup("OPT1    ", &id) == DGK_OK && id == DGK_TYPE_OPT1);
+  CHECK(dgk_optr_kernel_setup("HIP", &id) == DGK_OK && id == DGK_TYPE_HIP);
+  CHECK(dgk_optr_kernel_setup("OPT2", &id) == DGK_ERR_OPTTYPE);
+  CHECK(std::strlen(dgk_last_error()) > 0);
+  CHECK(dgk_optr_kernel_setup(nullptr, &id) != DGK_OK);
+  CHECK(dgk_optr_kernel_setup("HIP", nullptr) != DGK_OK);
+}
+
+static void operators(const char *dir) {
+  for (int p = 1; p <= 15; ++p) {
+    const int Nq = p + 1, Np = Nq * Nq * Nq;
+    std::vector<double> x(Nq), D(Nq * Nq), L((size_t)Np * 6);
+    CHECK(dgk_read_operator(dir, p, x.data(), D.data(), L.data()) == DGK_OK);
+    // the LGL nodes end at -1 and 1; D annihilates constants
+    CHECK(x[0] == -1.0 && x[Nq - 1] == 1.0);
+    for (int i = 0; i < Nq; ++i) {
+      double s = 0.0;
+      for (int l = 0; l < Nq; ++l) s += D[i + Nq * l];
+      CHECK(std::fabs(s) < 1e-9);
+    }
+    // the lossless lift packing reproduces every coefficient
+    std::vector<double> packed(dgk::lift_delta_doubles(Nq));
+    CHECK(dgk::pack_lift_deltas(Nq, L.data(), packed.data()));
+    // the mixed form's tables from the row-major operator
+    std::vector<double> drow(Nq * Nq), d32((size_t)Nq * Nq + 3 * Nq);
+    for (int k = 0; k < Nq; ++k)
+      for (int l = 0; l < Nq; ++l) drow[k * Nq + l] = D[k + Nq * l];
+    dgk::split_tables(Nq, drow.data(), d32.data());
+    std::vector<double> xl(Nq);
+    dgk::lgl_nodes(Nq, xl.data());
+    for (int i = 0; i < Nq; ++i) CHECK(std::fabs(xl[i] - x[i]) < 1e-13);
+    if (Nq == 16) {
+      std::vector<double> fit(dgk::kMfmaFitDoubles);
+      dgk::mfma_fit_tables(drow.data(), fit.data());
+    }
+    // a lift that mixes signs in a group does not pack
+    std::vector<double> bad = L;
+    bad[Np * 1 + (Nq - 1)] = -bad[Np * 1 + (Nq - 1)];
+    if (bad[Np * 1 + (Nq - 1)] != 0.0) CHECK(!dgk::pack_lift_deltas(Nq, bad.data(), packed.data()));
+  }
+  std::vector<double> x(2), D(4), L(48);
+  CHECK(dgk_read_operator(dir, 0, x.data(), D.data(), L.data()) == DGK_ERR_ORDER);
+  CHECK(dgk_read_operator(dir, 16, x.data(), D.data(), L.data()) == DGK_ERR_ORDER);
+  CHECK(dgk_read_operator("/nonexistent-dir", 1, x.data(), D.data(), L.data()) == DGK_ERR_IO);
+}
+
+static void mesh(const char *dir, int NeX, int NeY, int NeZ, int p, double dx, double dy,
+                 double dz) {
+  int32_t sz[7];
+  CHECK(dgk_mesh_sizes(NeX, NeY, NeZ, p, sz) == DGK_OK);
+  const long long Np = sz[1], NfpTot = sz[3], Ne = sz[4], NeA = sz[5], Nh = sz[6];
+  const int Nq = sz[0];
+  std::vector<double> x(Nq), D(Nq * Nq), L((size_t)Np * 6);
+  CHECK(dgk_read_operator(dir, p, x.data(), D.data(), L.data()) == DGK_OK);
+  std::vector<int32_t> vm(NfpTot * Ne), vp(NfpTot * Ne), halo(Nh);
+  std::vector<double> nrm(NfpTot * Ne * 3), es(Np * Ne * 3), fs(NfpTot * Ne),
+      pos(Np * Ne * 3);
+  CHECK(dgk_mesh_setup(NeX, NeY, NeZ, p, dx, dy, dz, x.data(), vm.data(), vp.data(),
+                       nrm.data(), es.data(), fs.data(), pos.data(), halo.data()) == DGK_OK);
+  for (long long t = 0; t < NfpTot * Ne; ++t) {
+    CHECK(vm[t] >= 1 && vm[t] <= Np * Ne);
+    CHECK(vp[t] >= 1 && vp[t] <= Np * NeA);
+  }
+  for (long long t = 0; t < Nh; ++t) CHECK(halo[t] >= 1 && halo[t] <= Np * Ne);
+  // every output may be NULL
+  CHECK(dgk_mesh_setup(NeX, NeY, NeZ, p, dx, dy, dz, nullptr, vm.data(), nullptr, nullptr,
+                       nullptr, nullptr, nullptr, nullptr) == DGK_OK);
+}
+
+int main(int argc, char **argv) {
+  if (argc < 2) {
+    std::fprintf(stderr, "usage: %s operator_dir\n", argv[0]);
+    return 2;
+  }
+  const char *dir = argv[1];
+  selector();
+  operators(dir);
+  mesh(dir, 3, 4, 5, 3, 1.0, 1.0, 1.0);
+  mesh(dir, 1, 2, 1, 7, 2.0, 0.5, 1.5);
+  mesh(dir, 4, 1, 3, 15, 1.0, 3.0, 0.25);
+  mesh(dir, 2, 2, 2, 1, 1.0, 1.0, 1.0);
+  int32_t sz[7];
+  CHECK(dgk_mesh_sizes(0, 2, 2, 3, sz) == DGK_ERR_ARG);
+  CHECK(dgk_mesh_sizes(2, 2, 2, 16, sz) == DGK_ERR_ORDER);
+  CHECK(dgk_mesh_sizes(2, 2, 2, 3, nullptr) == DGK_ERR_ARG);
+  CHECK(dgk_mesh_sizes(1024, 1024, 1024, 7, sz) == DGK_ERR_ARG);  // past int32 indices
+  CHECK(dgk_mesh_setup(2, 2, 2, 3, 1.0, 1.0, 1.0, nullptr, nullptr, nullptr, nullptr, nullptr,
+                       nullptr, std::vector<double>(64 * 8 * 3).data(), nullptr) == DGK_ERR_ARG);
+  if (g_fail) {
+    std::fprintf(stderr, "%d checks failed\n", g_fail);
+    return 1;
+  }
+  std::printf("host checks passed\n");
+  return 0;
+}
