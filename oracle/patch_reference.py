@@ -1,7 +1,8 @@
 #!/usr/bin/env python3
 """Builds the reference program with the three edits a maintainer would make
-to select DGOptrKernel_OptType = "HIP" (the edits INTEGRATION.md lists), to
-show that they compile, link against libdgk.so and run.
+to select DGOptrKernel_OptType = "HIP" (a "HIP" arm in the selector, the halo
+map made public, the step loop handed to dgk_run), to show that they compile,
+link against libdgk.so and run.
 
     python oracle/patch_reference.py REF LIBDIR OUT
 

@@ -1,6 +1,6 @@
-"""The three edits INTEGRATION.md lists for the reference (a "HIP" arm in
-dg_optr_kernel_setup, the halo map made public, the step loop handed to
-dgk_run), applied by oracle/patch_reference.py to a copy of the reference's
+"""The three edits a maintainer makes to the reference to select the
+library (a "HIP" arm in dg_optr_kernel_setup, the halo map made public, the
+step loop handed to dgk_run), applied by oracle/patch_reference.py to a copy of the reference's
 own sources and compiled against libdgk.so and mod_dgk_hip into
 oracle/_ref/scale-dg_extraction_hip (built where the reference tree is
 present; the program travels like the library).
