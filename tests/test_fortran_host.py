@@ -58,7 +58,7 @@ def test_missing_operator_data_is_reported(tmp_path):
 @pytest.mark.gpu
 @pytest.mark.parametrize("name", sorted(CS.STOCK_RUNS))
 @pytest.mark.parametrize("connectivity", ["STRUCTURED", "TABLES"])
-def test_stdout_equals_reference_program(tmp_path, golden, name, connectivity):
+def test_stdout_equals_reference_program(gpu, tmp_path, golden, name, connectivity):
     cfg = CS.STOCK_RUNS[name]
     say(f"[fortran host] {name} {connectivity}")
     r = run_host(tmp_path, CS.namelist_text(cfg, opt="HIP"),
@@ -69,7 +69,7 @@ def test_stdout_equals_reference_program(tmp_path, golden, name, connectivity):
 
 
 @pytest.mark.gpu
-def test_final_field_equals_oracle(tmp_path):
+def test_final_field_equals_oracle(gpu, tmp_path):
     """The whole final state, not only min/max: the host dumps its initial
     and final owned q; the oracle run from the same initial q must end on
     the same bits."""

@@ -51,7 +51,7 @@ def test_reference_types_unchanged(tmp_path, golden, name):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("name", sorted(CS.STOCK_RUNS))
-def test_hip_type_prints_the_reference_lines(tmp_path, golden, name):
+def test_hip_type_prints_the_reference_lines(gpu, tmp_path, golden, name):
     cfg = CS.STOCK_RUNS[name]
     say(f"[patched reference] {name} HIP")
     r = run(tmp_path, cfg, "HIP")
