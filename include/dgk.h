@@ -163,7 +163,12 @@ int dgk_create_structured(dgk_solver **out, int NeX, int NeY, int NeZ, int p,
 int dgk_destroy(dgk_solver *s);
 
 /* Uploads q,u,v,w (Np,NeA) (owned parts used; halos are refilled on the
- * device, as main.f90:159-162 does after the initial condition). */
+ * device, as main.f90:159-162 does after the initial condition).
+ * Synchronous.  Fields above 16 MiB (here and in dgk_get_field, the
+ * ensemble's transfers and the stateless tendency) go through two pinned
+ * 64 MiB staging buffers per device, allocated on first use and held until
+ * the process ends, filled and emptied by up to 8 host threads of the
+ * library's own while the DMA of the other buffer runs. */
 int dgk_set_fields(dgk_solver *s, const double *q, const double *u,
                    const double *v, const double *w);
 /* Downloads one field (Np,NeA).  For q the halo slots hold the values of
