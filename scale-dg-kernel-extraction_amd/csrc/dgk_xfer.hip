@@ -47,7 +47,8 @@ int copy_threads() {
 }
 
 // Runs f(begin, end) over [0, n) split into copy_threads() contiguous parts
-// (one part, on the calling thread, below `serial` items).
+// (one part, on the calling thread, below `serial` items).  A part whose
+// thread cannot be started runs on the calling thread.
 template <class F>
 void par_for(size_t n, size_t serial, F f) {
   const int T = copy_threads();
@@ -61,7 +62,11 @@ void par_for(size_t n, size_t serial, F f) {
   for (int t = 1; t < T; ++t) {
     const size_t b = part * t;
     if (b >= n) break;
-    th.emplace_back([=] { f(b, std::min(n, b + part)); });
+    try {
+      th.emplace_back([=] { f(b, std::min(n, b + part)); });
+    } catch (...) {
+      f(b, std::min(n, b + part));
+    }
   }
   f(size_t(0), std::min(n, part));
   for (auto &x : th) x.join();
@@ -105,6 +110,41 @@ void par_memcpy(char *dst, const char *src, size_t n) {
   });
 }
 
+// The chunk loop of a staged host-to-device copy.
+int h2d_chunks(Staging *S, char *d, const char *s, size_t bytes, hipStream_t st) {
+  for (size_t off = 0, i = 0; off < bytes; off += kChunk, ++i) {
+    const size_t n = std::min(kChunk, bytes - off);
+    const int b = (int)(i & 1);
+    // the DMA out of this buffer two chunks ago has finished
+    if (i >= 2) XOK(hipEventSynchronize(S->ev[b]), "hipEventSynchronize");
+    par_memcpy(static_cast<char *>(S->buf[b]), s + off, n);
+    XOK(hipMemcpyAsync(d + off, S->buf[b], n, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+    XOK(hipEventRecord(S->ev[b], st), "hipEventRecord");
+  }
+  return DGK_OK;
+}
+
+// The chunk loop of a staged device-to-host copy: chunk i's DMA into buffer
+// i % 2 runs while chunk i-1 is copied out of the other buffer; buffer i % 2
+// was emptied at step i-1.
+int d2h_chunks(Staging *S, char *d, const char *s, size_t bytes, hipStream_t st) {
+  const size_t nchunk = (bytes + kChunk - 1) / kChunk;
+  for (size_t i = 0; i <= nchunk; ++i) {
+    if (i < nchunk) {
+      const size_t off = i * kChunk, n = std::min(kChunk, bytes - off);
+      XOK(hipMemcpyAsync(S->buf[i & 1], s + off, n, hipMemcpyDeviceToHost, st),
+          "hipMemcpyAsync");
+      XOK(hipEventRecord(S->ev[i & 1], st), "hipEventRecord");
+    }
+    if (i >= 1) {
+      const size_t j = i - 1, off = j * kChunk, n = std::min(kChunk, bytes - off);
+      XOK(hipEventSynchronize(S->ev[j & 1]), "hipEventSynchronize");
+      par_memcpy(d + off, static_cast<const char *>(S->buf[j & 1]), n);
+    }
+  }
+  return DGK_OK;
+}
+
 }  // namespace
 
 int find_not_equal(const double *a, size_t n, double v, size_t *where) {
@@ -133,19 +173,13 @@ int copy_h2d(void *dst, const void *src, size_t bytes, void *stream) {
   Staging *S = nullptr;
   int rc = staging(&S);
   if (rc) return rc;
-  const char *s = static_cast<const char *>(src);
-  char *d = static_cast<char *>(dst);
-  for (size_t off = 0, i = 0; off < bytes; off += kChunk, ++i) {
-    const size_t n = std::min(kChunk, bytes - off);
-    const int b = (int)(i & 1);
-    // the DMA out of this buffer two chunks ago has finished
-    if (i >= 2) XOK(hipEventSynchronize(S->ev[b]), "hipEventSynchronize");
-    par_memcpy(static_cast<char *>(S->buf[b]), s + off, n);
-    XOK(hipMemcpyAsync(d + off, S->buf[b], n, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
-    XOK(hipEventRecord(S->ev[b], st), "hipEventRecord");
-  }
-  XOK(hipStreamSynchronize(st), "hipStreamSynchronize");
-  return DGK_OK;
+  rc = h2d_chunks(S, static_cast<char *>(dst), static_cast<const char *>(src), bytes, st);
+  // every DMA out of the staging buffers has ended before they are reused,
+  // also when the loop stopped at an error
+  const hipError_t e = hipStreamSynchronize(st);
+  if (!rc && e != hipSuccess)
+    rc = fail(DGK_ERR_HIP, "hipStreamSynchronize: %s", hipGetErrorString(e));
+  return rc;
 }
 
 int copy_d2h(void *dst, const void *src, size_t bytes, void *stream) {
@@ -159,25 +193,10 @@ int copy_d2h(void *dst, const void *src, size_t bytes, void *stream) {
   Staging *S = nullptr;
   int rc = staging(&S);
   if (rc) return rc;
-  const char *s = static_cast<const char *>(src);
-  char *d = static_cast<char *>(dst);
-  const size_t nchunk = (bytes + kChunk - 1) / kChunk;
-  // chunk i's DMA into buffer i % 2 runs while chunk i-1 is copied out of
-  // the other buffer; buffer i % 2 was emptied at step i-1
-  for (size_t i = 0; i <= nchunk; ++i) {
-    if (i < nchunk) {
-      const size_t off = i * kChunk, n = std::min(kChunk, bytes - off);
-      XOK(hipMemcpyAsync(S->buf[i & 1], s + off, n, hipMemcpyDeviceToHost, st),
-          "hipMemcpyAsync");
-      XOK(hipEventRecord(S->ev[i & 1], st), "hipEventRecord");
-    }
-    if (i >= 1) {
-      const size_t j = i - 1, off = j * kChunk, n = std::min(kChunk, bytes - off);
-      XOK(hipEventSynchronize(S->ev[j & 1]), "hipEventSynchronize");
-      par_memcpy(d + off, static_cast<const char *>(S->buf[j & 1]), n);
-    }
-  }
-  return DGK_OK;
+  rc = d2h_chunks(S, static_cast<char *>(dst), static_cast<const char *>(src), bytes, st);
+  // no DMA into the staging buffers is left running, also after an error
+  if (rc) (void)hipStreamSynchronize(st);
+  return rc;
 }
 
 }  // namespace dgk
