@@ -582,9 +582,15 @@ __device__ __forceinline__ void pencil_compute(PT &P, const double *__restrict__
   //   y-line (la,l,lb): pidx(la + NQ l + NFP lb) = yb + (NQ + 1) l
   //   face slots: pidx(pl) = zb, pidx(i + NQ k) = i + (NQ + 1) k
   // For a power of two the division is a shift and pidx itself needs fewer
-  // live registers (Nq = 16: 166 against 171 VGPRs, three wavefronts per
-  // SIMD instead of two).
-  constexpr bool kShift = (NQ & (NQ - 1)) == 0;
+  // live registers, which counts where VGPRs set the occupancy: the
+  // one-wavefront stage and the lean stage reading q from memory (Nq = 16,
+  // uniform-velocity FMA form: 162 against 171 VGPRs, three wavefronts per
+  // SIMD instead of two).  The lean stage with q in LDS is held to two
+  // elements per CU at Nq = 16 by its LDS anyway, and there the closed forms
+  // let the compiler pair neighbouring reads into ds_read2_b64 (p=15,
+  // per-node velocity: 124 instead of 196 LDS instructions per wavefront,
+  // the stage 3-4 % faster).
+  constexpr bool kShift = (NQ & (NQ - 1)) == 0 && (QG || !LEAN);
   const int zb = pl + lb, xb = (NQ + 1) * pl, yb = la + (NFP + NQ) * lb;
   auto pz = [&](int k) { return kShift ? pidx<NQ>(pl + NFP * k) : zb + (NFP + NQ) * k; };
   auto px = [&](int l) { return kShift ? pidx<NQ>(l + NQ * pl) : xb + l; };
