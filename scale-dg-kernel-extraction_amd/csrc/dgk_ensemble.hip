@@ -140,7 +140,7 @@ __device__ __forceinline__ void ens_items(const EnsArgs &A, int first, int count
     const int pl = act ? tid - el * NFP : 0;
     long long e = first_el + el;
     if (!act || e >= P.Ne) e = -1;
-    pencil_element<NQ, ORDER, FMA, FACT, true, CV, true, CParams>(
+    pencil_element<NQ, ORDER, FMA, FACT, true, CV, true, CParams, false, true>(
         P, ORDER == kMixed ? O.Drow32 : O.Drow, O.LiftP, O.lfac,
         smem + el * PER, pl, e, first_el, act);
     __syncthreads();  // LDS is reused by the next item
