@@ -189,7 +189,9 @@ int dgk_cal_tend(dgk_solver *s);
 int dgk_rk_stage(dgk_solver *s, int stage, double dt);
 /* One full time step (three stages) replayed from a captured hipGraph. */
 int dgk_step(dgk_solver *s, double dt);
-/* nstep time steps; if minmax is non-NULL it receives (min,max) of the owned
+/* nstep time steps, each as three stage launches on the solver's stream (on
+ * small meshes faster than dgk_step's graph replays; the same bits); if
+ * minmax is non-NULL it receives (min,max) of the owned
  * q after every step, 2*nstep doubles (the main.f90:76-79 diagnostics), and
  * a state with a NaN or an infinity makes the call return DGK_ERR_NONFINITE
  * (naming the first such step) after all nstep steps and minmax are done; a

@@ -478,7 +478,7 @@ def test_constant_velocity_form(gpu, p, factored):
 
 def test_graph_step_equals_stagewise(gpu):
     """Replays of the captured time step, with eager work and host syncs in
-    between, equal the stage-by-stage launches bit for bit."""
+    between, equal the stage-by-stage launches and dgk_run bit for bit."""
     ne, p = (4, 3, 5), 5
     x1D, D, L = dgk.read_operator(p)
     m = dgk.mesh_setup(*ne, p, x1D=x1D, tables=False)
@@ -498,8 +498,15 @@ def test_graph_step_equals_stagewise(gpu):
         b.synchronize()
         _ = b.cal_tend()  # eager work between replays
         assert np.array_equal(a.get_field(), b.get_field()), it
+    # dgk_run launches the stages on the stream: the same bits as the replays
+    c = dgk.Solver.structured(*ne, p, D, L)
+    c.set_fields(q, *vel)
+    c.run(dt, 2)
+    c.run(dt, 3)
+    assert np.array_equal(a.get_field(), c.get_field())
     a.close()
     b.close()
+    c.close()
 
 
 def test_q0_placement_and_stage_sequence(gpu):
