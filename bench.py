@@ -844,13 +844,13 @@ def main():
     del q, u, v, w
     Ne, Np = s.Ne, s.Np
 
-    for _ in range(args.warmup):
-        s.step(dt)
+    # the reference program's step loop (dgk_run: three stage launches per
+    # step on the solver's stream; it returns once the steps are done)
+    s.run(dt, args.warmup)
     s.synchronize()
     log(f"warmup done ({args.warmup} steps)")
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        s.step(dt)
+    s.run(dt, args.steps)
     s.synchronize()
     el = time.perf_counter() - t0
     dof = Ne * Np
