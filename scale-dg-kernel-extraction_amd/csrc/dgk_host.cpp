@@ -191,13 +191,19 @@ bool pack_lift_deltas(int Nq, const double *Lift, double *out) {
   auto *bytes = reinterpret_cast<uint8_t *>(out + 6 * Nq);
   std::memset(bytes, 0, (size_t)Np * 8);
   for (int pl = 0; pl < Nfp; ++pl)
-    for (int k = 0; k < Nq; ++k)
+    for (int k = 0; k < Nq; ++k) {
+      // the words of nodes 2r and 2r+1 of a line side by side; an odd Nq's
+      // last node alone
+      const size_t word = (Nq % 2 == 1 && k == Nq - 1)
+                              ? (size_t)k * Nfp + pl
+                              : (size_t)(k & ~1) * Nfp + 2 * (size_t)pl + (k & 1);
       for (int f = 0; f < 6; ++f) {
         const int i = pl % Nq, j = pl / Nq;
         const int64_t d =
             bits(Lift[pl + Nfp * k + (size_t)Np * f]) - base[f * Nq + lift_normal_index(f, i, j, k)];
-        bytes[((size_t)k * Nfp + pl) * 8 + f] = (uint8_t)d;
+        bytes[word * 8 + f] = (uint8_t)d;
       }
+    }
   return true;
 }
 

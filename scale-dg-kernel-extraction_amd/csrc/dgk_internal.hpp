@@ -42,8 +42,10 @@ void lgl_nodes(int Nq, double *x);
 // 4 and 5), so that a lane reads 8 bytes per node instead of 6 doubles.
 // out (lift_delta_doubles(Nq) doubles): the 6 Nq bases (as doubles, group
 // f Nq + n: the group's smallest bit pattern), then one 8-byte word per
-// node, word k Nq^2 + pl for node k of line pl = i + Nq j, its byte f the
-// delta of Lift(i,j,k,f) (bytes 6, 7 zero).  False (out unspecified) when a
+// node, its byte f the delta of Lift(i,j,k,f) (bytes 6, 7 zero): for node k
+// of line pl = i + Nq j word (k - k%2) Nq^2 + 2 pl + k%2, so that the words
+// of nodes 2r and 2r+1 of a line are one 16-byte load, except for the last
+// node of an odd Nq, word (Nq-1) Nq^2 + pl.  False (out unspecified) when a
 // group mixes signs, holds a non-finite value or spans more than 255 units
 // in the last place.
 constexpr size_t lift_delta_doubles(int Nq) { return 6 * (size_t)Nq + (size_t)Nq * Nq * Nq; }

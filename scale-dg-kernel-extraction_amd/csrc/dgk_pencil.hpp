@@ -664,19 +664,26 @@ __device__ __forceinline__ void pencil_compute(PT &P, const double *__restrict__
   };
   // Dense lift, delta-packed (pack_lift_deltas): Lift(i,j,k,f) is the double
   // whose bits are those of the base of (f, normal index) plus the unsigned
-  // byte f of this line's 8-byte word for node k, word [k NFP + pl] (lanes
-  // contiguous).  The bases of the four side faces are fixed along the line;
-  // those of -z and +z follow k and are the same for every lane.  The words
-  // of the lane's Nq nodes are requested together ahead of the final phase:
-  // loaded inside its loop, between the stores, each node waited on its own
-  // cache round trip.
+  // byte f of this line's 8-byte word for node k.  The words of nodes 2r and
+  // 2r+1 of a line are adjacent, words [2r NFP + 2 pl] and [.. + 1] (lanes
+  // contiguous, one 16-byte load per pair), and an odd Nq's last node has
+  // word [(NQ-1) NFP + pl].  The bases of the four side faces are fixed along
+  // the line; those of -z and +z follow k and are the same for every lane.
+  // The words of the lane's Nq nodes are requested together ahead of the
+  // final phase: loaded inside its loop, between the stores, each node
+  // waited on its own cache round trip.
   const long long *LB = reinterpret_cast<const long long *>(Lift);
   const uint2 *LW = reinterpret_cast<const uint2 *>(Lift + 6 * NQ);
   uint2 lw[FACT_LIFT ? 1 : NQ];
   auto load_lw = [&] {
     if constexpr (!FACT_LIFT) {
 #pragma unroll
-      for (int k = 0; k < NQ; ++k) lw[k] = LW[k * NFP + pl];
+      for (int r = 0; r < NQ / 2; ++r) {
+        const uint4 two = *reinterpret_cast<const uint4 *>(LW + 2 * r * NFP + 2 * pl);
+        lw[2 * r] = make_uint2(two.x, two.y);
+        lw[2 * r + 1] = make_uint2(two.z, two.w);
+      }
+      if constexpr (NQ % 2 == 1) lw[NQ - 1] = LW[(NQ - 1) * NFP + pl];
     }
   };
   // Factored lift: the coefficients of the four side faces are constant

@@ -48,6 +48,26 @@ static void operators(const char *dir) {
     // the lossless lift packing reproduces every coefficient
     std::vector<double> packed(dgk::lift_delta_doubles(Nq));
     CHECK(dgk::pack_lift_deltas(Nq, L.data(), packed.data()));
+    {
+      // base of (face, normal index) plus byte f of the node's word, the
+      // words of nodes 2r and 2r+1 of a line side by side
+      const int Nfp = Nq * Nq;
+      const auto *bytes = reinterpret_cast<const unsigned char *>(packed.data() + 6 * Nq);
+      for (int f = 0; f < 6; ++f)
+        for (int k = 0; k < Nq; ++k)
+          for (int j = 0; j < Nq; ++j)
+            for (int i = 0; i < Nq; ++i) {
+              const int pl = i + Nq * j;
+              const int n = f == 0 || f == 2 ? j : f == 1 || f == 3 ? i : k;
+              const size_t word = (Nq % 2 == 1 && k == Nq - 1)
+                                      ? (size_t)k * Nfp + pl
+                                      : (size_t)(k - k % 2) * Nfp + 2 * (size_t)pl + k % 2;
+              long long b, want;
+              std::memcpy(&b, &packed[f * Nq + n], sizeof b);
+              std::memcpy(&want, &L[i + Nq * j + (size_t)Nfp * k + (size_t)Np * f], sizeof want);
+              CHECK(b + bytes[word * 8 + f] == want);
+            }
+    }
     // the mixed form's tables from the row-major operator
     std::vector<double> drow(Nq * Nq), d32((size_t)Nq * Nq + 3 * Nq);
     for (int k = 0; k < Nq; ++k)
