@@ -821,7 +821,7 @@ int finish_create(dgk_solver *s, const double *D1D, const double *Lift) {
   const int Nq = s->Nq, Np = s->Np;
   const size_t nq2 = (size_t)Nq * Nq, nlift = (size_t)6 * Np;
   if ((rc = dalloc(&s->D, nq2)) || (rc = dalloc(&s->Lift, nlift)) ||
-      (rc = dalloc(&s->lfac, 2 * (size_t)Nq)) || (rc = dalloc(&s->Drow, nq2)) ||
+      (rc = dalloc(&s->lfac, 4 * (size_t)Nq)) || (rc = dalloc(&s->Drow, nq2)) ||
       (rc = dalloc(&s->LiftP, lift_delta_doubles(Nq))) ||
       (rc = dalloc(&s->Drow32, nq2 + 3 * Nq + (Nq == 16 ? kMfmaFitDoubles : 0))))
     return rc;
@@ -841,10 +841,16 @@ int finish_create(dgk_solver *s, const double *D1D, const double *Lift) {
     for (int l = 0; l < Nq; ++l) dr[(size_t)k * Nq + l] = D1D[k + (size_t)l * Nq];
   HIPOK(hipMemcpy(s->Drow, dr.data(), nq2 * sizeof(double), hipMemcpyHostToDevice));
   s->ops.Drow = s->Drow;
-  std::vector<double> lf(2 * (size_t)Nq);
+  // [l_-, l_+], then the pairs (l_-(n), l_+(n)) that the one-wavefront stage
+  // reads with one 16-byte load per index
+  std::vector<double> lf(4 * (size_t)Nq);
   for (int i = 0; i < Nq; ++i) {
     lf[i] = Lift[i + (size_t)Np * 3];       // face 4 (-x): l_-(i) = Lift(i,1,1,4)
     lf[Nq + i] = Lift[i + (size_t)Np * 1];  // face 2 (+x): l_+(i) = Lift(i,1,1,2)
+  }
+  for (int i = 0; i < Nq; ++i) {
+    lf[2 * Nq + 2 * i] = lf[i];
+    lf[2 * Nq + 2 * i + 1] = lf[Nq + i];
   }
   HIPOK(hipMemcpy(s->lfac, lf.data(), lf.size() * sizeof(double), hipMemcpyHostToDevice));
   s->ops.lfac = s->lfac;

@@ -9,7 +9,8 @@ namespace dgk {
 
 // Device copies of the 1D operator in row-major order (D(k,l) at
 // [k*Nq + l], read a row at a time by the pencil kernel) and of the vectors
-// [l_-, l_+] of the factored lift.
+// [l_-, l_+] of the factored lift, followed by the pairs (l_-(n), l_+(n)) for
+// n < Nq (4 Nq doubles; the one-wavefront stage reads the pairs).
 struct HostOps {
   const double *Drow;
   const double *lfac;

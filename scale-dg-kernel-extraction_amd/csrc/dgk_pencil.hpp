@@ -687,14 +687,17 @@ __device__ __forceinline__ void pencil_compute(PT &P, const double *__restrict__
     }
   };
   // Factored lift: the coefficients of the four side faces are constant
-  // along the lane's line, l_-(j), l_+(i), l_+(j), l_-(i) (non-lean stage).
+  // along the lane's line, l_-(j), l_+(i), l_+(j), l_-(i) (non-lean stage),
+  // read as the pairs (l_-(n), l_+(n)) that follow [l_-, l_+] in lfac.
   double lside[4] = {0.0, 0.0, 0.0, 0.0};
   auto load_lside = [&] {
     if constexpr (FACT_LIFT) {
-      lside[0] = lfac[lb];
-      lside[1] = lfac[NQ + la];
-      lside[2] = lfac[NQ + lb];
-      lside[3] = lfac[la];
+      const double2 *pairs = reinterpret_cast<const double2 *>(lfac + 2 * NQ);
+      const double2 pj = pairs[lb], pi = pairs[la];
+      lside[0] = pj.x;
+      lside[1] = pi.y;
+      lside[2] = pj.y;
+      lside[3] = pi.x;
     }
   };
   // Non-lean stage at Nq >= 4: the final phase's lift data is requested
